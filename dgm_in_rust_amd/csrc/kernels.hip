@@ -1,0 +1,1063 @@
+// HIP kernels (gfx950) of the DG-Euler explicit step and their host-side launch sequences.
+//
+// Per RK stage (src/temporal_disc/rk3.rs:7-26):
+//   k_surface  one lane per edge: traces of both neighbours at the Ne edge points, HLLC
+//              (src/spatial_disc/flux.rs:28-153) on interior edges, slip-wall / characteristic
+//              far-field flux (src/spatial_disc/boundary_condition.rs:9-97) on boundary edges;
+//              writes flux * |e|/2 per point. Each edge is evaluated once with the reference's
+//              left/right orientation (the altered p* makes HLLC orientation-dependent).
+//   k_element  one lane per element: volume integral (src/spatial_disc.rs:40-69) with reference-
+//              element gradients, gather of its three edges' fluxes and lift, the diagonal mass
+//              inverse (src/spatial_disc.rs:349-357), and the SSP-RK3 stage combination. The
+//              flux-to-element step is a gather, so no atomics and no edge colouring are needed.
+//   limiter    KXRCF detector (src/spatial_disc/limiter/weno.rs:7-85), one lane per element,
+//              then one wavefront per troubled element for the characteristic WENO rebuild
+//              (weno.rs:86-269) with FP64 smoothness indicators and weights.
+// Per step: k_dt min-reduction (src/temporal_disc.rs:58-88) and device-side step control.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <memory>
+#include <vector>
+
+#include "kernels.h"
+#include "solver.h"
+
+namespace dgm {
+
+namespace {
+
+constexpr int np_of(int p) { return (p + 1) * (p + 2) / 2; }
+
+// ---- arithmetic helpers: FP64 uses IEEE operations; the FP32 bulk path uses the hardware
+// reciprocal / square root / log2 / exp2 (1 ulp class), which is its stated precision. ----
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double sqrt_(double a) { return sqrt(a); }
+__device__ __forceinline__ float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double pow_(double a, double b) { return pow(a, b); }
+__device__ __forceinline__ float pow_(float a, float b) { return __builtin_amdgcn_exp2f(b * __builtin_amdgcn_logf(a)); }
+
+template <class T>
+__device__ __forceinline__ const T* geo_el(const DevMesh& m);
+template <>
+__device__ __forceinline__ const double* geo_el<double>(const DevMesh& m) { return m.el_geo; }
+template <>
+__device__ __forceinline__ const float* geo_el<float>(const DevMesh& m) { return m.el_geo32; }
+template <class T>
+__device__ __forceinline__ const T* geo_ed(const DevMesh& m);
+template <>
+__device__ __forceinline__ const double* geo_ed<double>(const DevMesh& m) { return m.ed_geo; }
+template <>
+__device__ __forceinline__ const float* geo_ed<float>(const DevMesh& m) { return m.ed_geo32; }
+
+// Table offsets (bulk type T): PHIV, DXI, DETA [NQ][NP]; PHIE, LIFT [3][NE][NP].
+template <int NP, int NQ, int NE>
+struct TabOff {
+  static constexpr int PHIV = 0, DXI = NQ * NP, DETA = 2 * NQ * NP, PHIE = 3 * NQ * NP, LIFT = 3 * NQ * NP + 3 * NE * NP;
+  static constexpr int N = 3 * NQ * NP + 6 * NE * NP;
+};
+// FP64 limiter tables: PHIE [3][NE][NP], EW [NE], CW [NQ], DREF [nkeys][NQ][NP]
+template <int P, int NQ, int NE>
+struct LimOff {
+  static constexpr int NP = np_of(P);
+  static constexpr int MAXD = P > 2 ? P : 2;
+  static constexpr int NKEYS = MAXD * (MAXD + 3) / 2;
+  static constexpr int PHIE = 0, EW = 3 * NE * NP, CW = EW + NE, DREF = CW + NQ;
+  static constexpr int N = DREF + NKEYS * NQ * NP;
+  static constexpr int NKW = P * (P + 3) / 2;  // physical keys (l1, l - l1), l = 1..P
+};
+__host__ __device__ constexpr int dref_key(int a, int b) { return (a + b) * (a + b + 1) / 2 - 1 + b; }
+
+// ---- physics ----
+
+// flux::flux, src/spatial_disc/flux.rs:154-168
+template <class T>
+__device__ __forceinline__ void euler_flux(const T q[4], T hcr, T f[4], T g[4]) {
+  const T ri = dv(T(1), q[0]);
+  const T u = q[1] * ri, v = q[2] * ri;
+  const T p = (hcr - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) * ri);
+  f[0] = q[1];
+  f[1] = q[1] * u + p;
+  f[2] = q[2] * u;
+  f[3] = u * (q[3] + p);
+  g[0] = q[2];
+  g[1] = q[1] * v;
+  g[2] = q[2] * v + p;
+  g[3] = v * (q[3] + p);
+}
+
+// flux::hllc, src/spatial_disc/flux.rs:28-153, including the reference's p* numerator
+// (c_L + c_R - ((zeta - 1)/2)(u_R - u_L)) of :75-78. Returns 0, or 1 / 2 for a negative density /
+// pressure; the pressure test runs in FP64 whatever T is.
+template <class T>
+__device__ __forceinline__ int hllc(const T lv[4], const T rv[4], T nx, T ny, T hcr, T out[4]) {
+  const T ql0 = lv[0], ql1 = lv[1] * nx + lv[2] * ny, ql2 = -lv[1] * ny + lv[2] * nx, ql3 = lv[3];
+  const T qr0 = rv[0], qr1 = rv[1] * nx + rv[2] * ny, qr2 = -rv[1] * ny + rv[2] * nx, qr3 = rv[3];
+  const T il = dv(T(1), ql0), ir = dv(T(1), qr0);
+  const T ul = ql1 * il, ur = qr1 * ir, vl = ql2 * il, vr = qr2 * ir;
+  const T pl = (hcr - T(1)) * (ql3 - T(0.5) * (ql1 * ql1 + ql2 * ql2) * il);
+  const T pr = (hcr - T(1)) * (qr3 - T(0.5) * (qr1 * qr1 + qr2 * qr2) * ir);
+  if (ql0 < T(0) || qr0 < T(0)) return 1;
+  {
+    const double h = (double)hcr;
+    const double pl64 = (h - 1.0) * ((double)ql3 - 0.5 * ((double)ql1 * (double)ql1 + (double)ql2 * (double)ql2) / (double)ql0);
+    const double pr64 = (h - 1.0) * ((double)qr3 - 0.5 * ((double)qr1 * (double)qr1 + (double)qr2 * (double)qr2) / (double)qr0);
+    if (pl64 < 0.0 || pr64 < 0.0) return 2;
+  }
+  const T cl = sqrt_(hcr * pl * il), cr = sqrt_(hcr * pr * ir);
+  const T zeta = dv(hcr - T(1), T(2) * hcr);
+  const T base = dv(cl + cr - (zeta - T(1)) * T(0.5) * (ur - ul), dv(cl, pow_(pl, zeta)) + dv(cr, pow_(pr, zeta)));
+  const T ps = pow_(base, dv(T(1), zeta));
+  const T gk = dv(hcr + T(1), T(2) * hcr);
+  const T qlw = (ps <= pl) ? T(1) : sqrt_(T(1) + gk * (dv(ps, pl) - T(1)));
+  const T qrw = (ps <= pr) ? T(1) : sqrt_(T(1) + gk * (dv(ps, pr) - T(1)));
+  const T sl = ul - cl * qlw, sr = ur + cr * qrw;
+  T f0, fn, ft, f3;
+  if (sl >= T(0)) {
+    f0 = ql1;
+    fn = ql1 * ul + pl;
+    ft = ql2 * ul;
+    f3 = ul * (ql3 + pl);
+  } else if (sr <= T(0)) {
+    f0 = qr1;
+    fn = qr1 * ur + pr;
+    ft = qr2 * ur;
+    f3 = ur * (qr3 + pr);
+  } else {
+    const T ss = dv(pr - pl + ql0 * ul * (sl - ul) - qr0 * ur * (sr - ur), ql0 * (sl - ul) - qr0 * (sr - ur));
+    if (ss >= T(0)) {
+      const T a = dv(ql0 * (sl - ul), sl - ss);
+      const T e3 = a * (ql3 * il + (ss - ul) * (ss + dv(pl, ql0 * (sl - ul))));
+      f0 = ql1 + sl * (a - ql0);
+      fn = (ql1 * ul + pl) + sl * (a * ss - ql1);
+      ft = (ql2 * ul) + sl * (a * vl - ql2);
+      f3 = ul * (ql3 + pl) + sl * (e3 - ql3);
+    } else {
+      const T a = dv(qr0 * (sr - ur), sr - ss);
+      const T e3 = a * (qr3 * ir + (ss - ur) * (ss + dv(pr, qr0 * (sr - ur))));
+      f0 = qr1 + sr * (a - qr0);
+      fn = (qr1 * ur + pr) + sr * (a * ss - qr1);
+      ft = (qr2 * ur) + sr * (a * vr - qr2);
+      f3 = ur * (qr3 + pr) + sr * (e3 - qr3);
+    }
+  }
+  out[0] = f0;
+  out[1] = fn * nx - ft * ny;
+  out[2] = fn * ny + ft * nx;
+  out[3] = f3;
+  return 0;
+}
+
+// local_characteristics::compute_eigenmatrix, src/spatial_disc/local_characteristics.rs:4-50
+template <class T>
+__device__ __forceinline__ void eigenmatrix(const T q[4], T nx, T ny, T hcr, T L[16], T R[16]) {
+  const T ri = dv(T(1), q[0]);
+  const T u = q[1] * ri, v = q[2] * ri;
+  const T p = (hcr - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) * ri);
+  const T c = sqrt_(hcr * p * ri);
+  const T ic = dv(T(1), c);
+  const T h = (q[3] + p) * ri;
+  const T b1 = (hcr - T(1)) * ic * ic;
+  const T b2 = b1 * (u * u + v * v) * T(0.5);
+  const T vn = u * nx + v * ny;
+  L[0] = (b2 + vn * ic) * T(0.5);
+  L[1] = -(b1 * u + nx * ic) * T(0.5);
+  L[2] = -(b1 * v + ny * ic) * T(0.5);
+  L[3] = b1 * T(0.5);
+  L[4] = ny * u - nx * v;
+  L[5] = -ny;
+  L[6] = nx;
+  L[7] = T(0);
+  L[8] = T(1) - b2;
+  L[9] = b1 * u;
+  L[10] = b1 * v;
+  L[11] = -b1;
+  L[12] = (b2 - vn * ic) * T(0.5);
+  L[13] = -(b1 * u - nx * ic) * T(0.5);
+  L[14] = -(b1 * v - ny * ic) * T(0.5);
+  L[15] = b1 * T(0.5);
+  R[0] = T(1);
+  R[1] = T(0);
+  R[2] = T(1);
+  R[3] = T(1);
+  R[4] = u - c * nx;
+  R[5] = -ny;
+  R[6] = u;
+  R[7] = u + c * nx;
+  R[8] = v - c * ny;
+  R[9] = nx;
+  R[10] = v;
+  R[11] = v + c * ny;
+  R[12] = h - c * vn;
+  R[13] = -u * ny + v * nx;
+  R[14] = (u * u + v * v) * T(0.5);
+  R[15] = h + c * vn;
+}
+
+// characteristic far-field state, src/spatial_disc/boundary_condition.rs:48-94
+template <class T>
+__device__ __forceinline__ void farfield_flux(const T lv[4], const double* qinf, T nx, T ny, T hcr, T out[4]) {
+  const T ri = dv(T(1), lv[0]);
+  const T u = lv[1] * ri, v = lv[2] * ri;
+  const T p = (hcr - T(1)) * (lv[3] - T(0.5) * (lv[1] * lv[1] + lv[2] * lv[2]) * ri);
+  const T c = sqrt_(hcr * p * ri);
+  const T vn = u * nx + v * ny;
+  T L[16], R[16];
+  eigenmatrix(lv, nx, ny, hcr, L, R);
+  const T lam[4] = {vn - c, vn, vn, vn + c};
+  const T qi[4] = {(T)qinf[0], (T)qinf[1], (T)qinf[2], (T)qinf[3]};
+  T w[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const T a = L[4 * i] * lv[0] + L[4 * i + 1] * lv[1] + L[4 * i + 2] * lv[2] + L[4 * i + 3] * lv[3];
+    const T b = L[4 * i] * qi[0] + L[4 * i + 1] * qi[1] + L[4 * i + 2] * qi[2] + L[4 * i + 3] * qi[3];
+    w[i] = lam[i] >= T(0) ? a : b;
+  }
+  T q[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) q[i] = R[4 * i] * w[0] + R[4 * i + 1] * w[1] + R[4 * i + 2] * w[2] + R[4 * i + 3] * w[3];
+  const T qi0 = dv(T(1), q[0]);
+  const T un = q[1] * qi0, vnw = q[2] * qi0;
+  const T vnn = un * nx + vnw * ny;
+  const T pn = (hcr - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) * qi0);
+  out[0] = q[0] * vnn;
+  out[1] = q[1] * vnn + pn * nx;
+  out[2] = q[2] * vnn + pn * ny;
+  out[3] = (q[3] + pn) * vnn;
+}
+
+__device__ __forceinline__ void record_error(Ctl* ctl, int edge, int code) {
+  atomicMin(&ctl->error, ((unsigned long long)(unsigned)edge << 8) | (unsigned long long)code);
+}
+
+// ---------------------------------------------------------------------------------------
+// surface kernel: one lane per edge
+// ---------------------------------------------------------------------------------------
+template <class T, int NP, int NQ, int NE>
+__global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict__ U, T* __restrict__ G,
+                                                 const T* __restrict__ tab, Ctl* __restrict__ ctl, int force) {
+  using O = TabOff<NP, NQ, NE>;
+  __shared__ T sphi[3 * NE * NP];
+  for (int i = threadIdx.x; i < 3 * NE * NP; i += blockDim.x) sphi[i] = tab[O::PHIE + i];
+  __syncthreads();
+  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
+  const int ed = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ed >= m.nedge) return;
+  const int nes = m.nes;
+  const int info = m.ed_info[ed];
+  const int licL = info & 3, licR = (info >> 2) & 3, bc = (info >> 4) & 3, patch = info >> 8;
+  const int L = m.ed_elem[ed];
+  const T* eg = geo_ed<T>(m);
+  const T nx = eg[ed], ny = eg[m.nedge + ed], hl = eg[2 * m.nedge + ed];
+  const T hcr = (T)ctl->hcr;
+  T ul[NE][4];
+#pragma unroll
+  for (int q = 0; q < NE; ++q)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) ul[q][v] = T(0);
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const T c = U[(size_t)(v * NP + k) * nes + L];
+#pragma unroll
+      for (int q = 0; q < NE; ++q) ul[q][v] += c * sphi[(licL * NE + q) * NP + k];
+    }
+  T* g = G + (size_t)ed * NE * 4;
+  if (bc == BC_INTERIOR) {
+    const int R = m.ed_elem[m.nedge + ed];
+    T ur[NE][4];
+#pragma unroll
+    for (int q = 0; q < NE; ++q)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) ur[q][v] = T(0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const T c = U[(size_t)(v * NP + k) * nes + R];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) ur[q][v] += c * sphi[(licR * NE + (NE - 1 - q)) * NP + k];
+      }
+    int err = 0;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      T f[4];
+      const int s = hllc(ul[q], ur[q], nx, ny, hcr, f);
+      if (s && !err) err = s;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) g[q * 4 + v] = f[v] * hl;
+    }
+    if (err) record_error(ctl, ed, err);
+  } else if (bc == BC_WALL) {
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const T p = (hcr - T(1)) * (ul[q][3] - T(0.5) * dv(ul[q][1] * ul[q][1] + ul[q][2] * ul[q][2], ul[q][0]));
+      g[q * 4 + 0] = T(0);
+      g[q * 4 + 1] = p * nx * hl;
+      g[q * 4 + 2] = p * ny * hl;
+      g[q * 4 + 3] = T(0);
+    }
+  } else {
+    const double* qinf = m.patch_q + 4 * patch;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      T f[4];
+      farfield_flux(ul[q], qinf, nx, ny, hcr, f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) g[q * 4 + v] = f[v] * hl;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// element kernel: one lane per element — volume integral + flux lift + mass inverse + RK stage
+// ---------------------------------------------------------------------------------------
+template <class T, int NP, int NQ, int NE, int OP>
+__global__ __launch_bounds__(128) void k_element(DevMesh m, StageBufs<T> b, const T* __restrict__ G,
+                                                 const T* __restrict__ tab, Ctl* __restrict__ ctl, int force) {
+  using O = TabOff<NP, NQ, NE>;
+  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0) {
+    // the limiter of this stage counts troubled cells from zero
+    const int slot = OP == OP_S0 ? 0 : (OP == OP_S1 ? 1 : 2);
+    ctl->count[slot] = 0;
+  }
+  if (e >= m.ne) return;
+  const int nes = m.nes;
+  T u[4][NP];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int k = 0; k < NP; ++k) u[v][k] = b.in[(size_t)(v * NP + k) * nes + e];
+  const T* eg = geo_el<T>(m);
+  const T xx = eg[G_XX * nes + e], xy = eg[G_XY * nes + e], ex = eg[G_EX * nes + e], ey = eg[G_EY * nes + e];
+  const T j2 = eg[G_2J * nes + e];
+  const T hcr = (T)ctl->hcr;
+  T r[4][NP];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int k = 0; k < NP; ++k) r[v][k] = T(0);
+  // volume integral with reference-element gradients: sum_q (F~ dphi/dxi + G~ dphi/deta) w_q / m_k
+#pragma unroll 2
+  for (int q = 0; q < NQ; ++q) {
+    T w[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < NP; ++k) s += u[v][k] * tab[O::PHIV + q * NP + k];
+      w[v] = s;
+    }
+    T f[4], g[4];
+    euler_flux(w, hcr, f, g);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const T ft = xx * f[v] + xy * g[v];
+      const T gt = ex * f[v] + ey * g[v];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) r[v][k] += ft * tab[O::DXI + q * NP + k] + gt * tab[O::DETA + q * NP + k];
+    }
+  }
+  // surface: gather the three edges' point fluxes (reversed point order on the right side)
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int es = m.el_edge[i * nes + e];
+    const int edge = es >> 1, side = es & 1;
+    const T* g = G + (size_t)edge * NE * 4;
+    const T sg = side ? j2 : -j2;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      const int src = side ? (NE - 1 - q) : q;
+      T gv[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) gv[v] = sg * g[src * 4 + v];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int k = 0; k < NP; ++k) r[v][k] += gv[v] * tab[O::LIFT + (i * NE + q) * NP + k];
+    }
+  }
+  const size_t base = e;
+  if (OP == OP_RESID) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int k = 0; k < NP; ++k) b.rout[(size_t)(v * NP + k) * nes + base] = (double)r[v][k];
+    return;
+  }
+  const double dt = ctl->dt;
+  if constexpr (sizeof(T) == 8) {
+    // src/temporal_disc/rk3.rs:10,16,22
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const size_t ix = (size_t)(v * NP + k) * nes + base;
+        const double rr = (double)r[v][k], uu = (double)u[v][k];
+        if (OP == OP_S0)
+          b.us[ix] = uu + dt * rr;
+        else if (OP == OP_S1)
+          b.us[ix] = 0.75 * b.un[ix] + 0.25 * (uu + dt * rr);
+        else
+          b.un[ix] = 1.0 / 3.0 * b.un[ix] + 2.0 / 3.0 * (uu + dt * rr);
+      }
+  } else {
+    // FP32-bulk in increment form: u^(1) = u^n + d1, d1 = dt R(u^n); u^(2) = u^n + d2,
+    // d2 = (d1 + dt R(u^(1))) / 4; u^(n+1) = u^n + 2 (d2 + dt R(u^(2))) / 3 accumulated in FP64.
+    const float dtf = (float)dt;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const size_t ix = (size_t)(v * NP + k) * nes + base;
+        const float rr = (float)r[v][k];
+        if (OP == OP_S0) {
+          const float d = dtf * rr;
+          b.d[ix] = d;
+          b.s[ix] = (float)u[v][k] + d;
+        } else if (OP == OP_S1) {
+          const float d = 0.25f * (b.d[ix] + dtf * rr);
+          b.d[ix] = d;
+          b.s[ix] = b.sn[ix] + d;
+        } else {
+          const double d = 2.0 / 3.0 * ((double)b.d[ix] + dt * (double)rr);
+          const double un = b.un[ix] + d;
+          b.un[ix] = un;
+          b.sn[ix] = (float)un;
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// time step: TemperalDisc::compute_time_step (wave speed from the sum of ALL modes, Q5)
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <class T, int NP>
+__global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, Ctl* __restrict__ ctl) {
+  __shared__ double red[4];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double dt = 1.0e10;
+  if (e < m.ne) {
+    const int nes = m.nes;
+    double s[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double a = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) a += (double)U[(size_t)(v * NP + k) * nes + e];
+      s[v] = a;
+    }
+    const double hcr = ctl->hcr;
+    const double u = s[1] / s[0], v = s[2] / s[0];
+    const double p = (hcr - 1.0) * (s[3] - 0.5 * s[0] * (u * u + v * v));
+    const double c = sqrt(hcr * p / s[0]);
+    const double speed = sqrt(u * u + v * v) + c;
+    const double d = ctl->cfl * m.el_geo[G_HMIN * nes + e] / speed;
+    if (d < dt) dt = d;  // a NaN candidate is never taken, as in the reference's `if dt < time_step`
+  }
+  dt = wave_min(dt);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) b = fmin(b, red[i]);
+    atomicMin(&ctl->dtmin_bits, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+// while step < final_step && t < final_time (src/temporal_disc.rs:22-26)
+__global__ void k_step_begin(Ctl* ctl) {
+  const double dtm = __longlong_as_double((long long)ctl->dtmin_bits);
+  ctl->dtmin_bits = (unsigned long long)__double_as_longlong(1.0e10);
+  const bool go = ctl->step < ctl->final_step && ctl->t < ctl->final_time && ctl->error == ~0ull;
+  ctl->active = go ? 1 : 0;
+  double dt = dtm;
+  if (ctl->t + dt > ctl->final_time) dt = ctl->final_time - ctl->t;
+  if (go) ctl->dt = dt;
+}
+
+__global__ void k_step_end(Ctl* ctl) {
+  if (ctl->active && ctl->error == ~0ull) {
+    ctl->t += ctl->dt;
+    ctl->step += 1;
+    ctl->last_count = ctl->count[2];
+  }
+}
+
+__global__ void k_reset_count(Ctl* ctl, int slot) { ctl->count[slot] = 0; }
+
+// ---------------------------------------------------------------------------------------
+// limiter
+// ---------------------------------------------------------------------------------------
+
+// modified_kxrcf, src/spatial_disc/limiter/weno.rs:7-85. The neighbour trace uses the target's own
+// local edge index with reversed point order (:21-27, :63-64); faces without a neighbour are skipped;
+// no inflow face gives 0/0 = NaN, which is not >= 1.
+template <class T, int P, int NQ, int NE>
+__device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, int e) {
+  using LO = LimOff<P, NQ, NE>;
+  constexpr int NP = LO::NP;
+  const int nes = m.nes;
+  double rho[NP], ru[NP], rv[NP], en[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    rho[k] = (double)U[(size_t)(0 * NP + k) * nes + e];
+    ru[k] = (double)U[(size_t)(1 * NP + k) * nes + e];
+    rv[k] = (double)U[(size_t)(2 * NP + k) * nes + e];
+    en[k] = (double)U[(size_t)(3 * NP + k) * nes + e];
+  }
+  double djump = 0.0, ejump = 0.0, surface = 0.0, minr = 1.0e10, mine = 1.0e10;
+  const double* phie = lt + LO::PHIE;
+  const double* ew = lt + LO::EW;
+#pragma unroll
+  for (int lic = 0; lic < 3; ++lic) {
+    const int nb = m.el_nbr[lic * nes + e];
+    if (nb < 0) continue;
+    const int es = m.el_edge[lic * nes + e];
+    const int edge = es >> 1;
+    const double sgn = (es & 1) ? -1.0 : 1.0;
+    const double nx = sgn * m.ed_geo[edge], ny = sgn * m.ed_geo[m.nedge + edge];
+    const double len = 2.0 * m.ed_geo[2 * m.nedge + edge];
+    double tr[NE], te[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const double ph = phie[(lic * NE + q) * NP + k];
+        a += rho[k] * ph;
+        b += en[k] * ph;
+      }
+      tr[q] = a;
+      te[q] = b;
+      if (a < minr) minr = a;
+      if (b < mine) mine = b;
+    }
+    double mu = 0.0, mv = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const double ph = phie[(lic * NE + NE / 2) * NP + k];
+      mu += ru[k] * ph;
+      mv += rv[k] * ph;
+    }
+    if (mu * nx + mv * ny < 0.0) {
+      double rj = 0.0, ej = 0.0;
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        double xr = 0.0, xe = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          const double ph = phie[(lic * NE + (NE - 1 - q)) * NP + k];
+          xr += (double)U[(size_t)(0 * NP + k) * nes + nb] * ph;
+          xe += (double)U[(size_t)(3 * NP + k) * nes + nb] * ph;
+        }
+        rj += 0.5 * len * ew[q] * (tr[q] - xr);
+        ej += 0.5 * len * ew[q] * (te[q] - xe);
+      }
+      djump += rj;
+      ejump += ej;
+      surface += len;
+    }
+  }
+  const double h15 = pow(m.el_geo[G_CR * nes + e], 1.5);
+  const double id = fabs(djump) / (h15 * surface * minr);
+  const double ie = fabs(ejump) / (h15 * surface * mine);
+  return (id >= 1.0) || (ie >= 1.0);
+}
+
+template <class T, int P, int NQ, int NE>
+__global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ U, const double* __restrict__ lt,
+                                               LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force) {
+  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e);
+  if (lb.flags) lb.flags[e] = t ? 1 : 0;
+  if (t) {
+    const int i = atomicAdd(&ctl->count[slot], 1);
+    lb.list[i] = e;
+  }
+}
+
+// WENO rebuild of one troubled element by one 64-lane wavefront (block of 64), FP64 throughout.
+// Lane (face i, polynomial j, variable v) = 16 i + 4 j + v projects polynomial j (0: the target,
+// 1..3: the neighbour across face j-1 with its mean replaced by the target's, weno.rs:148-155) on
+// the characteristic row v of face i (eigenmatrix at the target mean and the raw edge normal,
+// weno.rs:121-124), forms beta = sum_l J^(l-1) sum_{l1} int (d^l P / dx^l1 dy^(l-l1))^2 as P^T B P with
+// the element's smoothness matrix B, and the nonlinear weight linear_w / (beta + eps)^2.
+template <int P, int NQ, int NE>
+struct WenoShared {
+  static constexpr int NP = np_of(P);
+  static constexpr int NKW = P * (P + 3) / 2;
+  double src[4][4][NP];
+  double D[NKW][NQ][NP];
+  double B[NP][NP];
+  double Pj[3][4][4][NP];
+  double nlw[3][4][4];
+  double Rrow[3][4][4];
+  double PN[3][4][NP];
+  double RES[3][4][NP];
+  double area[3];
+  int nb[3];
+  int edge[3];
+  int flag;
+};
+
+template <class T, int P, int NQ, int NE>
+__device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, double hcr, int e,
+                         WenoShared<P, NQ, NE>& sh, double* __restrict__ out /* [4][NP], global */) {
+  using LO = LimOff<P, NQ, NE>;
+  constexpr int NP = LO::NP;
+  constexpr int NKW = LO::NKW;
+  const int lane = threadIdx.x;
+  const int nes = m.nes;
+  if (lane < 3) {
+    const int nb = m.el_nbr[lane * nes + e];
+    sh.nb[lane] = nb;
+    sh.edge[lane] = m.el_edge[lane * nes + e] >> 1;
+    sh.area[lane] = nb >= 0 ? m.el_geo[G_J * nes + nb] * 0.5 : 0.0;
+  }
+  __syncthreads();
+  for (int idx = lane; idx < 16 * NP; idx += 64) {
+    const int j = idx / (4 * NP), v = (idx / NP) % 4, k = idx % NP;
+    double val = 0.0;
+    if (j == 0 || k == 0)
+      val = (double)U[(size_t)(v * NP + k) * nes + e];
+    else if (sh.nb[j - 1] >= 0)
+      val = (double)U[(size_t)(v * NP + k) * nes + sh.nb[j - 1]];
+    if (j > 0 && sh.nb[j - 1] < 0) val = 0.0;
+    sh.src[j][v][k] = val;
+  }
+  const double J = m.el_geo[G_J * nes + e];
+  const double xx = m.el_geo[G_XX * nes + e], xy = m.el_geo[G_XY * nes + e];
+  const double ex = m.el_geo[G_EX * nes + e], ey = m.el_geo[G_EY * nes + e];
+  const double* dref = lt + LO::DREF;
+  // physical derivatives d^l phi_k / dx^l1 dy^(l-l1) at the cell points (src/mesh.rs:101-112 for l <= 2;
+  // binomial expansion of (xx d_xi + ex d_eta)^l1 (xy d_xi + ey d_eta)^l2 in general)
+  for (int idx = lane; idx < NKW * NQ * NP; idx += 64) {
+    const int kw = idx / (NQ * NP), q = (idx / NP) % NQ, k = idx % NP;
+    int l = 1;
+    while ((l + 1) * (l + 2) / 2 - 1 <= kw) ++l;
+    const int l1 = kw - (l * (l + 1) / 2 - 1), l2 = l - l1;
+    double s = 0.0;
+    double cx = 1.0;  // C(l1, i)
+    for (int i = 0; i <= l1; ++i) {
+      double cy = 1.0;  // C(l2, jj)
+      for (int jj = 0; jj <= l2; ++jj) {
+        double c = cx * cy;
+        for (int t = 0; t < i; ++t) c *= xx;
+        for (int t = 0; t < l1 - i; ++t) c *= ex;
+        for (int t = 0; t < jj; ++t) c *= xy;
+        for (int t = 0; t < l2 - jj; ++t) c *= ey;
+        const int a = i + jj, bb = l - i - jj;
+        s += c * dref[(dref_key(a, bb) * NQ + q) * NP + k];
+        cy = cy * (l2 - jj) / (jj + 1);
+      }
+      cx = cx * (l1 - i) / (i + 1);
+    }
+    sh.D[kw][q][k] = s;
+  }
+  __syncthreads();
+  // smoothness matrix B[k][k'] = sum_l J^(l-1) sum_l1 sum_q (J/2) w_q D[q][k] D[q][k']
+  const double* cw = lt + LO::CW;
+  for (int idx = lane; idx < NP * NP; idx += 64) {
+    const int k = idx / NP, kk = idx % NP;
+    double s = 0.0;
+    for (int kw = 0; kw < NKW; ++kw) {
+      int l = 1;
+      while ((l + 1) * (l + 2) / 2 - 1 <= kw) ++l;
+      double jl = 1.0;
+      for (int t = 0; t < l - 1; ++t) jl *= J;
+      double a = 0.0;
+      for (int q = 0; q < NQ; ++q) a += 0.5 * J * cw[q] * sh.D[kw][q][k] * sh.D[kw][q][kk];
+      s += jl * a;
+    }
+    sh.B[k][kk] = s;
+  }
+  // linear weights, weno.rs:93-110 (a boundary face i zeroes weights[i])
+  double lw[4] = {0.997, 0.001, 0.001, 0.001};
+  int nbnd = 0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (sh.nb[i] < 0) {
+      nbnd += 1;
+      lw[i] = 0.0;
+    }
+  if (nbnd == 0)
+    lw[0] = 0.997;
+  else if (nbnd == 1)
+    lw[1] = 0.998;
+  else if (nbnd == 2)
+    lw[2] = 0.999;
+  __syncthreads();
+  if (nbnd > 2) return 5;  // the reference panics: "Invalid number of boundary neighbours!"
+  const double EPS = 2.220446049250313e-16;
+  if (lane < 48) {
+    const int i = lane >> 4, j = (lane >> 2) & 3, v = lane & 3;
+    double Pk[NP];
+    double w = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) Pk[k] = 0.0;
+    if (sh.nb[i] >= 0) {
+      const double nx = m.ed_geo[sh.edge[i]], ny = m.ed_geo[m.nedge + sh.edge[i]];
+      const double mean[4] = {sh.src[0][0][0], sh.src[0][1][0], sh.src[0][2][0], sh.src[0][3][0]};
+      double Lm[16], Rm[16];
+      eigenmatrix(mean, nx, ny, hcr, Lm, Rm);
+      if (j == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh.Rrow[i][v][c] = Rm[4 * v + c];
+      }
+      if (j == 0 || sh.nb[j - 1] >= 0) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k)
+          Pk[k] = Lm[4 * v] * sh.src[j][0][k] + Lm[4 * v + 1] * sh.src[j][1][k] + Lm[4 * v + 2] * sh.src[j][2][k] +
+                  Lm[4 * v + 3] * sh.src[j][3][k];
+        double beta = 0.0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          double t = 0.0;
+#pragma unroll
+          for (int kk = 0; kk < NP; ++kk) t += sh.B[k][kk] * Pk[kk];
+          beta += Pk[k] * t;
+        }
+        w = lw[j] / ((beta + EPS) * (beta + EPS));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) sh.Pj[i][j][v][k] = Pk[k];
+    sh.nlw[i][j][v] = w;
+  }
+  __syncthreads();
+  if (lane < 12) {
+    const int i = lane >> 2, v = lane & 3;
+    if (sh.nb[i] >= 0) {
+      const double den = ((sh.nlw[i][0][v] + sh.nlw[i][1][v]) + sh.nlw[i][2][v]) + sh.nlw[i][3][v];
+      double wj[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) wj[j] = sh.nlw[i][j][v] / den;
+#pragma unroll
+      for (int k = 0; k < NP; ++k)
+        sh.PN[i][v][k] = wj[0] * sh.Pj[i][0][v][k] + wj[1] * sh.Pj[i][1][v][k] + wj[2] * sh.Pj[i][2][v][k] +
+                         wj[3] * sh.Pj[i][3][v][k];
+    }
+  }
+  __syncthreads();
+  if (lane < 12) {
+    const int i = lane >> 2, v = lane & 3;
+    if (sh.nb[i] >= 0) {
+#pragma unroll
+      for (int k = 0; k < NP; ++k)
+        sh.RES[i][v][k] = sh.Rrow[i][v][0] * sh.PN[i][0][k] + sh.Rrow[i][v][1] * sh.PN[i][1][k] +
+                          sh.Rrow[i][v][2] * sh.PN[i][2][k] + sh.Rrow[i][v][3] * sh.PN[i][3][k];
+    }
+  }
+  __syncthreads();
+  // area-weighted mean over faces, weno.rs:111-118,196,260-263
+  for (int idx = lane; idx < 4 * NP; idx += 64) {
+    const int v = idx / NP, k = idx % NP;
+    double rec = 0.0, area = 0.0;
+    for (int i = 0; i < 3; ++i)
+      if (sh.nb[i] >= 0) {
+        rec += sh.RES[i][v][k] * sh.area[i];
+        area += sh.area[i];
+      }
+    out[idx] = rec / area;
+  }
+  return 0;
+}
+
+// store a rebuilt element (double [4][NP]) into the buffers of the given stage
+template <class T, int NP, int STAGE>
+__device__ __forceinline__ void store_limited(const StageBufs<T>& b, int nes, int e, int idx, double val) {
+  const int v = idx / NP, k = idx % NP;
+  const size_t ix = (size_t)(v * NP + k) * nes + e;
+  if constexpr (sizeof(T) == 8) {
+    if (STAGE == 2)
+      b.un[ix] = val;
+    else
+      b.us[ix] = (T)val;
+  } else {
+    if (STAGE == 2) {
+      b.un[ix] = val;
+      b.sn[ix] = (T)val;
+    } else {
+      b.s[ix] = (T)val;
+      b.d[ix] = (T)(val - b.un[ix]);
+    }
+  }
+}
+
+template <class T, int P, int NQ, int NE>
+__global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U, const double* __restrict__ lt,
+                                             LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force) {
+  __shared__ WenoShared<P, NQ, NE> sh;
+  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
+  constexpr int NP = np_of(P);
+  const int n = ctl->count[slot];
+  for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    const int e = lb.list[s];
+    const int st = weno_wave<T, P, NQ, NE>(m, U, lt, ctl->hcr, e, sh, lb.w + (size_t)s * 4 * NP);
+    if (st && threadIdx.x == 0) atomicMin(&ctl->error, (0xFFFFFFull << 8) | (unsigned long long)st);
+    __syncthreads();
+  }
+}
+
+template <class T, int NP, int STAGE>
+__global__ __launch_bounds__(256) void k_apply(LimiterBufs lb, StageBufs<T> b, int nes, Ctl* __restrict__ ctl, int slot,
+                                               int force) {
+  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
+  const int n = ctl->count[slot];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * 4 * NP; t += gridDim.x * blockDim.x) {
+    const int s = t / (4 * NP), idx = t % (4 * NP);
+    store_limited<T, NP, STAGE>(b, nes, lb.list[s], idx, lb.w[(size_t)s * 4 * NP + idx]);
+  }
+}
+
+// exact element-order sweep: one wavefront per element of one level; elements of a level are not
+// adjacent, so each reads final values of lower-numbered neighbours and untouched higher ones.
+template <class T, int P, int NQ, int NE, int STAGE>
+__global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __restrict__ elems, int n, StageBufs<T> b,
+                                                    const T* __restrict__ U, const double* __restrict__ lt,
+                                                    LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force) {
+  __shared__ WenoShared<P, NQ, NE> sh;
+  __shared__ double outv[4 * np_of(P)];
+  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
+  constexpr int NP = np_of(P);
+  const int s = blockIdx.x;
+  if (s >= n) return;
+  const int e = elems[s];
+  if (threadIdx.x == 0) {
+    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e);
+    sh.flag = t ? 1 : 0;
+    if (lb.flags) lb.flags[e] = t ? 1 : 0;
+  }
+  __syncthreads();
+  if (!sh.flag) return;
+  const int st = weno_wave<T, P, NQ, NE>(m, U, lt, ctl->hcr, e, sh, outv);
+  __syncthreads();
+  if (st) {
+    if (threadIdx.x == 0) atomicMin(&ctl->error, ((unsigned long long)(unsigned)e << 8) | (unsigned long long)st);
+    return;
+  }
+  for (int idx = threadIdx.x; idx < 4 * NP; idx += 64) store_limited<T, NP, STAGE>(b, m.nes, e, idx, outv[idx]);
+  if (threadIdx.x == 0) atomicAdd(&ctl->count[slot], 1);
+}
+
+// ---------------------------------------------------------------------------------------
+// layout conversion and checks
+// ---------------------------------------------------------------------------------------
+// host layout [ne][4][NP] (double) -> device [4 NP][nes]
+template <class T>
+__global__ void k_to_soa(const double* __restrict__ in, T* __restrict__ out, double* __restrict__ out64, int ne, int nes,
+                         int ncomp) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (e >= ne) return;
+  const double v = in[(size_t)e * ncomp + c];
+  if (out) out[(size_t)c * nes + e] = (T)v;
+  if (out64) out64[(size_t)c * nes + e] = v;
+}
+__global__ void k_from_soa(const double* __restrict__ in, double* __restrict__ out, int ne, int nes, int ncomp) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (e >= ne) return;
+  out[(size_t)e * ncomp + c] = in[(size_t)c * nes + e];
+}
+__global__ void k_copy_f64(const double* __restrict__ in, double* __restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in[i];
+}
+__global__ void k_check_finite(const double* __restrict__ u, int ne, int nes, int ncomp, int* flag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  for (int c = 0; c < ncomp; ++c)
+    if (!isfinite(u[(size_t)c * nes + e])) {
+      atomicOr(flag, 1);
+      return;
+    }
+}
+
+// wall pressure force (Fx, Fy) from the mode coefficients: sum over wall edges and edge points of
+// p * n * w * |e| / 2, with the normal oriented out of the fluid element
+template <class T, int P, int NQ, int NE>
+__global__ void k_wall_force(DevMesh m, const double* __restrict__ U, const double* __restrict__ lt, double hcr,
+                             double* __restrict__ force) {
+  using LO = LimOff<P, NQ, NE>;
+  constexpr int NP = LO::NP;
+  const int ed = blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0.0, fy = 0.0;
+  if (ed < m.nedge) {
+    const int info = m.ed_info[ed];
+    if (((info >> 4) & 3) == BC_WALL) {
+      const int L = m.ed_elem[ed], lic = info & 3;
+      const double nx = m.ed_geo[ed], ny = m.ed_geo[m.nedge + ed], hl = m.ed_geo[2 * m.nedge + ed];
+      for (int q = 0; q < NE; ++q) {
+        double s[4] = {0, 0, 0, 0};
+        for (int v = 0; v < 4; ++v)
+          for (int k = 0; k < NP; ++k) s[v] += U[(size_t)(v * NP + k) * m.nes + L] * lt[LO::PHIE + (lic * NE + q) * NP + k];
+        const double p = (hcr - 1.0) * (s[3] - 0.5 * (s[1] * s[1] + s[2] * s[2]) / s[0]);
+        fx += p * nx * lt[LO::EW + q] * hl;
+        fy += p * ny * lt[LO::EW + q] * hl;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    fx += __shfl_xor(fx, o, 64);
+    fy += __shfl_xor(fy, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0 && (fx != 0.0 || fy != 0.0)) {
+    atomicAdd(&force[0], fx);
+    atomicAdd(&force[1], fy);
+  }
+}
+
+inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+void launch_step_begin(Ctl* ctl, hipStream_t st) { k_step_begin<<<1, 1, 0, st>>>(ctl); }
+void launch_step_end(Ctl* ctl, hipStream_t st) { k_step_end<<<1, 1, 0, st>>>(ctl); }
+
+// ---------------------------------------------------------------------------------------
+// Launch sequences for one (precision, order, rule) combination
+// ---------------------------------------------------------------------------------------
+template <class T, int P, int NQ, int NE>
+class SolverImpl final : public Solver {
+ public:
+  static constexpr int NP = np_of(P);
+  using O = TabOff<NP, NQ, NE>;
+  using LO = LimOff<P, NQ, NE>;
+
+  int table_len() const override { return O::N; }
+  int limiter_table_len() const override { return LO::N; }
+  bool fp32() const override { return sizeof(T) == 4; }
+
+  StageBufs<T> bufs(const DeviceState& s) const {
+    StageBufs<T> b;
+    b.in = nullptr;
+    b.un = s.un;
+    b.us = sizeof(T) == 8 ? (T*)s.us : nullptr;
+    b.sn = sizeof(T) == 4 ? (T*)s.sn : nullptr;
+    b.s = sizeof(T) == 4 ? (T*)s.s : nullptr;
+    b.d = sizeof(T) == 4 ? (T*)s.d : nullptr;
+    b.rout = s.rout;
+    return b;
+  }
+  // stage input: FP64 reads u^n then the stage state; FP32-bulk reads the float shadows
+  const T* stage_input(const DeviceState& s, int stage) const {
+    if (sizeof(T) == 8) return stage == 0 ? (const T*)s.un : (const T*)s.us;
+    return stage == 0 ? (const T*)s.sn : (const T*)s.s;
+  }
+  const T* limiter_input(const DeviceState& s, int stage) const {
+    if (sizeof(T) == 8) return stage == 2 ? (const T*)s.un : (const T*)s.us;
+    return stage == 2 ? (const T*)s.sn : (const T*)s.s;
+  }
+
+  void surface(const DeviceState& s, const T* in, hipStream_t st, int force) const {
+    k_surface<T, NP, NQ, NE><<<blocks_for(s.mesh.nedge, 256), 256, 0, st>>>(s.mesh, in, (T*)s.flux, (const T*)s.tab,
+                                                                           s.ctl, force);
+  }
+  template <int OP>
+  void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force) const {
+    k_element<T, NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, 128), 128, 0, st>>>(s.mesh, b, (const T*)s.flux,
+                                                                            (const T*)s.tab, s.ctl, force);
+  }
+
+  void residual(const DeviceState& s, hipStream_t st) const override {
+    StageBufs<T> b = bufs(s);
+    b.in = stage_input(s, 0);
+    surface(s, b.in, st, 1);
+    element<OP_RESID>(s, b, st, 1);
+  }
+
+  template <int STAGE>
+  void limiter_stage(const DeviceState& s, hipStream_t st, int force, int slot) const {
+    StageBufs<T> b = bufs(s);
+    const T* U = limiter_input(s, STAGE);
+    LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags};
+    if (s.limiter == DGM_LIMITER_JACOBI) {
+      k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
+      k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
+      k_apply<T, NP, STAGE><<<s.apply_grid, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot, force);
+    } else if (s.limiter == DGM_LIMITER_SEQUENTIAL) {
+      for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
+        const int n = s.level_offsets[l + 1] - s.level_offsets[l];
+        k_limit_level<T, P, NQ, NE, STAGE><<<n, 64, 0, st>>>(s.mesh, s.levels + s.level_offsets[l], n, b, U, s.ltab,
+                                                             lb, s.ctl, slot, force);
+      }
+    }
+  }
+
+  void limiter(const DeviceState& s, hipStream_t st) const override {
+    // stand-alone SpatialDisc::apply_limiter on the resident u^n
+    k_reset_count<<<1, 1, 0, st>>>(s.ctl, 3);
+    limiter_stage<2>(s, st, 1, 3);
+  }
+
+  void stage(const DeviceState& s, int stage_idx, hipStream_t st) const override {
+    StageBufs<T> b = bufs(s);
+    b.in = stage_input(s, stage_idx);
+    surface(s, b.in, st, 0);
+    if (stage_idx == 0) {
+      element<OP_S0>(s, b, st, 0);
+      limiter_stage<0>(s, st, 0, 0);
+    } else if (stage_idx == 1) {
+      element<OP_S1>(s, b, st, 0);
+      limiter_stage<1>(s, st, 0, 1);
+    } else {
+      element<OP_S2>(s, b, st, 0);
+      limiter_stage<2>(s, st, 0, 2);
+    }
+  }
+
+  void time_step(const DeviceState& s, hipStream_t st) const override {
+    const T* U = sizeof(T) == 8 ? (const T*)s.un : (const T*)s.sn;
+    k_dt<T, NP><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ctl);
+  }
+
+  void upload(const DeviceState& s, const double* host_layout_dev, hipStream_t st) const override {
+    dim3 g(blocks_for(s.mesh.ne, 256), 4 * NP);
+    if (sizeof(T) == 8)
+      k_to_soa<double><<<g, 256, 0, st>>>(host_layout_dev, nullptr, s.un, s.mesh.ne, s.mesh.nes, 4 * NP);
+    else
+      k_to_soa<float><<<g, 256, 0, st>>>(host_layout_dev, (float*)s.sn, s.un, s.mesh.ne, s.mesh.nes, 4 * NP);
+  }
+  void download(const DeviceState& s, const double* soa, double* host_layout_dev, hipStream_t st) const override {
+    dim3 g(blocks_for(s.mesh.ne, 256), 4 * NP);
+    k_from_soa<<<g, 256, 0, st>>>(soa, host_layout_dev, s.mesh.ne, s.mesh.nes, 4 * NP);
+  }
+  void check_finite(const DeviceState& s, int* flag, hipStream_t st) const override {
+    k_check_finite<<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.un, s.mesh.ne, s.mesh.nes, 4 * NP, flag);
+  }
+  void wall_force(const DeviceState& s, double* force, hipStream_t st) const override {
+    k_wall_force<T, P, NQ, NE><<<blocks_for(s.mesh.nedge, 256), 256, 0, st>>>(s.mesh, s.un, s.ltab, s.hcr, force);
+  }
+};
+
+std::unique_ptr<Solver> make_solver(int precision, int p, int nq, int ne) {
+  const bool f32 = precision == DGM_FP32_BULK;
+#define DGM_CASE(P_, NQ_, NE_)                                               \
+  if (p == P_ && nq == NQ_ && ne == NE_) {                                   \
+    if (f32) return std::unique_ptr<Solver>(new SolverImpl<float, P_, NQ_, NE_>()); \
+    return std::unique_ptr<Solver>(new SolverImpl<double, P_, NQ_, NE_>());         \
+  }
+  DGM_CASE(1, 6, 3)
+  DGM_CASE(2, 6, 3)
+  DGM_CASE(3, 16, 5)
+  DGM_CASE(4, 25, 5)
+#undef DGM_CASE
+  return nullptr;
+}
+
+}  // namespace dgm

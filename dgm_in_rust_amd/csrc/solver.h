@@ -1,0 +1,62 @@
+// Launch-sequence interface shared by the C ABI (dgm.cpp) and the kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "../../include/dgm.h"
+#include "kernels.h"
+
+namespace dgm {
+
+// Device buffers of one context. Sizes: state arrays [4 Np][nes]; flux [n_edges][Ne][4].
+struct DeviceState {
+  DevMesh mesh{};
+  double* un = nullptr;    // u^n, FP64
+  void* us = nullptr;      // FP64 mode: stage state
+  void* sn = nullptr;      // FP32-bulk: float shadow of u^n
+  void* s = nullptr;       // FP32-bulk: float stage state
+  void* d = nullptr;       // FP32-bulk: stage increment
+  double* rout = nullptr;  // residual output
+  void* flux = nullptr;
+  void* tab = nullptr;     // bulk-precision tables
+  double* ltab = nullptr;  // FP64 limiter tables
+  int* lim_list = nullptr;
+  double* lim_w = nullptr;
+  uint8_t* lim_flags = nullptr;
+  Ctl* ctl = nullptr;
+  int limiter = 0;
+  unsigned weno_grid = 1, apply_grid = 1;
+  std::vector<int> level_offsets;
+  const int* levels = nullptr;
+  double hcr = 1.4;
+};
+
+class Solver {
+ public:
+  virtual ~Solver() = default;
+  virtual int table_len() const = 0;
+  virtual int limiter_table_len() const = 0;
+  virtual bool fp32() const = 0;
+  // residual of the resident u^n into rout (no time control, always runs)
+  virtual void residual(const DeviceState& s, hipStream_t st) const = 0;
+  // SpatialDisc::apply_limiter on the resident u^n (always runs)
+  virtual void limiter(const DeviceState& s, hipStream_t st) const = 0;
+  // RK stage 0, 1, 2 (surface + element + limiter), gated by Ctl::active
+  virtual void stage(const DeviceState& s, int stage_idx, hipStream_t st) const = 0;
+  // time-step reduction into Ctl::dtmin_bits
+  virtual void time_step(const DeviceState& s, hipStream_t st) const = 0;
+  virtual void upload(const DeviceState& s, const double* host_layout_dev, hipStream_t st) const = 0;
+  virtual void download(const DeviceState& s, const double* soa, double* host_layout_dev, hipStream_t st) const = 0;
+  virtual void check_finite(const DeviceState& s, int* flag, hipStream_t st) const = 0;
+  virtual void wall_force(const DeviceState& s, double* force, hipStream_t st) const = 0;
+};
+
+std::unique_ptr<Solver> make_solver(int precision, int p, int nq, int ne);
+
+// kernel launches with single-thread control
+void launch_step_begin(Ctl* ctl, hipStream_t st);
+void launch_step_end(Ctl* ctl, hipStream_t st);
+
+}  // namespace dgm

@@ -1,0 +1,166 @@
+/*
+ * dgm.h — C ABI of the MI355X (gfx950) DG-Euler explicit time step.
+ *
+ * Drop-in for the call surface of the Rust crate Decem5150/DGM_in_rust:
+ *   SpatialDisc::compute_residuals  src/spatial_disc.rs:27-39          -> dgm_compute_residuals
+ *   SpatialDisc::apply_limiter      src/spatial_disc/limiter.rs:7-9     -> dgm_apply_limiter
+ *   TemperalDisc::compute_time_step src/temporal_disc.rs:58-88          -> dgm_compute_time_step
+ *   TemperalDisc::rk3               src/temporal_disc/rk3.rs:7-26       -> dgm_rk3
+ *   TemperalDisc::time_march        src/temporal_disc.rs:21-57          -> dgm_time_march
+ *   TemperalDisc::set_residual_to_zero src/temporal_disc.rs:89-93       -> (no call: dgm_compute_residuals overwrites)
+ * and of the set-up that feeds them:
+ *   initialize_mesh_basis_gauss_params / process_mesh / Mesh::compute_*  src/initialization.rs:11-261,
+ *   src/mesh.rs:15-147, DubinerBasis::new src/basis_function.rs:15-38, GaussPoints::new src/gauss_point.rs:13-37
+ *                                                                       -> dgm_ctx_create
+ *   Solver::set_initial_solution    src/solver.rs:45-79                 -> dgm_initial_solution
+ *
+ * Host arrays use the reference layout: solutions / residuals are C-order [n_elements][4][n_basis]
+ * f64 (ndarray Array<f64, Ix3>, src/initialization.rs:169-170). All calls return a dgm_status;
+ * none aborts. Where the reference panics (negative density or pressure in HLLC,
+ * src/spatial_disc.rs:189-201; unsupported rule sizes, src/gauss_point.rs:56-58,90-92; more than
+ * two boundary faces in WENO, src/spatial_disc/limiter/weno.rs:107), the call returns the
+ * matching status and dgm_last_error names the offending edge or element.
+ *
+ * A context owns its device buffers and one HIP stream; it is not thread-safe. Different
+ * contexts may be used from different threads.
+ */
+#ifndef DGM_H
+#define DGM_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  DGM_OK = 0,
+  DGM_NEG_DENSITY = 1,   /* flux::hllc Err("Negative density found in HLLC!") */
+  DGM_NEG_PRESSURE = 2,  /* flux::hllc Err("Negative pressure found in HLLC!") */
+  DGM_NONFINITE = 3,     /* non-finite state found by dgm_check_state */
+  DGM_BAD_CONFIG = 4,    /* unsupported order / rule / option */
+  DGM_BAD_MESH = 5,      /* mesh the reference would panic on */
+  DGM_HIP_ERROR = 6,
+  DGM_OUT_OF_MEMORY = 7,
+  DGM_BAD_ARGUMENT = 8
+} dgm_status;
+
+enum { DGM_FP64 = 0, DGM_FP32_BULK = 1 };
+/* limiter sweep order: off; the reference's in-place element-order sweep (weno.rs:90,263), run on the
+ * device as a level schedule; or Jacobi (every cell tested and rebuilt from the pre-limiter state). */
+enum { DGM_LIMITER_OFF = 0, DGM_LIMITER_SEQUENTIAL = 1, DGM_LIMITER_JACOBI = 2 };
+enum { DGM_WALL = 0, DGM_FARFIELD = 1 };
+
+/* The mesh as GmshParser::parse returns it (src/io/mesh_parser.rs:28-158). Node ids are 0-based.
+ * element_xy (nullable) gives each triangle's own vertex coordinates [ne][3][2]: periodic meshes
+ * identify vertices across the period and need it (extension; the reference has no periodic BC). */
+typedef struct {
+  int64_t n_vertices;
+  const double* vertices_xy;            /* [n_vertices][2] */
+  int64_t n_boundary_blocks;
+  const int32_t* boundary_block_type;   /* DGM_WALL / DGM_FARFIELD, per block */
+  const int64_t* boundary_block_count;  /* lines per block */
+  const int64_t* boundary_nodes;        /* [sum counts][2] */
+  int64_t n_element_blocks;
+  const int64_t* element_block_count;   /* triangles per block */
+  const int64_t* element_nodes;         /* [sum counts][3], CCW */
+  const double* element_xy;             /* nullable */
+} dgm_mesh;
+
+/* parameters.json / freestream.json fields (src/io/param_parser.rs:6-19, boundary_condition_parser.rs:6-12). */
+typedef struct {
+  int32_t order;      /* p in 1..4 (the reference supports p = 2 only) */
+  int32_t n_cell_gp;  /* 0 = default: 6 for p <= 2, 16 for p = 3, 25 for p = 4 */
+  int32_t n_edge_gp;  /* 0 = default: 3 for p <= 2, 5 for p >= 3 */
+  double hcr;         /* gamma */
+  double cfl;         /* <= 0: 1/((2p+1)(1+4/(p+2)^2)), src/initialization.rs:128-132 */
+  double fs_mach, fs_aoa_deg, fs_rho, fs_p, fs_hcr; /* far-field state, src/initialization.rs:124-133 */
+  int32_t precision;  /* DGM_FP64 or DGM_FP32_BULK */
+  int32_t limiter;    /* DGM_LIMITER_* */
+} dgm_params;
+
+typedef struct {
+  int64_t n_elements, n_edges, n_internal_edges, n_internal_elements, n_boundary_elements;
+  int32_t n_basis, n_cell_gp, n_edge_gp, order, precision, limiter;
+  int32_t n_limiter_levels;  /* level-schedule depth of the sequential sweep */
+  double cfl;
+  int64_t device_bytes;      /* device memory held by the context */
+} dgm_info;
+
+typedef struct {
+  int32_t status;
+  int64_t edge;      /* offending edge (reference numbering) or -1 */
+  int64_t element;   /* offending element or -1 */
+  char message[256];
+} dgm_error;
+
+typedef struct {
+  int32_t steps;
+  double time;
+  double last_dt;
+  int64_t troubled_cells_last_stage;
+} dgm_march_stats;
+
+typedef struct dgm_ctx dgm_ctx;
+
+/* initialize_mesh_basis_gauss_params + initialize_solver (src/initialization.rs:11-111). */
+int dgm_ctx_create(const dgm_mesh* mesh, const dgm_params* params, dgm_ctx** out);
+/* Same set-up without touching the GPU: connectivity, geometry, tables and the projection helpers work;
+ * compute entry points return DGM_BAD_ARGUMENT. For host-side checks on machines without a GPU. */
+int dgm_ctx_create_host(const dgm_mesh* mesh, const dgm_params* params, dgm_ctx** out);
+void dgm_ctx_destroy(dgm_ctx* ctx);
+int dgm_ctx_info(const dgm_ctx* ctx, dgm_info* info);
+int dgm_last_error(const dgm_ctx* ctx, dgm_error* err);
+
+/* Connectivity in the reference's numbering (process_mesh, src/initialization.rs:112-261; Mesh::set_neighbours,
+ * src/mesh.rs:133-147). Any pointer may be NULL. Sizes: edges [n_edges][2] / [n_edges]; elements [ne][3]. */
+int dgm_get_connectivity(const dgm_ctx* ctx, int64_t* edge_elements, int64_t* edge_vertices, int64_t* edge_in_cell,
+                         int64_t* edge_patch, int64_t* element_vertices, int64_t* element_edges,
+                         int64_t* element_neighbours, int64_t* internal_edges, int64_t* internal_elements,
+                         int64_t* boundary_elements);
+/* Geometry (src/mesh.rs:15-84): edge normal [n_edges][2], edge length, element jacobian, circumradius, min height. */
+int dgm_get_geometry(const dgm_ctx* ctx, double* edge_normal, double* edge_length, double* element_jacob,
+                     double* element_circumradius, double* element_minimum_height);
+/* Basis and quadrature tables (DubinerBasis::new, GaussPoints::new). Any pointer may be NULL. */
+int dgm_get_tables(const dgm_ctx* ctx, double* phi_cell, double* phi_edge, double* mass, double* cell_points,
+                   double* cell_weights, double* edge_points, double* edge_weights);
+
+/* Solver::set_initial_solution (src/solver.rs:45-79): L2 projection of a uniform state into U (host). */
+int dgm_initial_solution(dgm_ctx* ctx, double density, double aoa_deg, double mach, double pressure, double* U);
+/* Extension: L2 projection of a state given at the cell quadrature points, q[ne][n_cell_gp][4]. */
+int dgm_project(dgm_ctx* ctx, const double* q_at_points, double* U);
+/* Physical coordinates of the cell quadrature points, xy[ne][n_cell_gp][2]. */
+int dgm_cell_points_xy(const dgm_ctx* ctx, double* xy);
+
+/* ---- host-buffer entry points (reference layout; each uploads, runs, downloads) ---- */
+/* SpatialDisc::compute_residuals. accumulate = 0: R = M^-1 (V - S - B) (overwrite; the caller need not zero R);
+ * accumulate = 1: the reference's semantics R = M^-1 (R_in + V - S - B). */
+int dgm_compute_residuals(dgm_ctx* ctx, double* residuals, const double* solutions, int32_t accumulate);
+/* SpatialDisc::apply_limiter; troubled (nullable, [ne] bytes) receives the cells that were rebuilt. */
+int dgm_apply_limiter(dgm_ctx* ctx, double* solutions, uint8_t* troubled);
+/* TemperalDisc::compute_time_step (wave speed from the sum of all modes, as the reference). */
+int dgm_compute_time_step(dgm_ctx* ctx, const double* solutions, double* dt);
+/* TemperalDisc::rk3: one SSP-RK3 step with the limiter after every stage. */
+int dgm_rk3(dgm_ctx* ctx, double* solutions, double dt);
+/* TemperalDisc::time_march: loop while step < final_step && t < final_time (dt clamped to final_time). */
+int dgm_time_march(dgm_ctx* ctx, double* solutions, int32_t final_step, double final_time, dgm_march_stats* stats);
+
+/* ---- device-resident state ---- */
+int dgm_state_upload(dgm_ctx* ctx, const double* solutions);
+int dgm_state_download(dgm_ctx* ctx, double* solutions);
+/* time_march on the resident state: steps run as replays of a captured HIP graph of one RK3 step. */
+int dgm_device_time_march(dgm_ctx* ctx, int32_t final_step, double final_time, dgm_march_stats* stats);
+/* enqueue n whole steps (time-step reduction + 3 stages + limiter) without synchronising. */
+int dgm_device_steps_async(dgm_ctx* ctx, int32_t n_steps);
+int dgm_synchronize(dgm_ctx* ctx);
+/* the context's HIP stream (hipStream_t), for event timing by the caller */
+void* dgm_stream(dgm_ctx* ctx);
+/* DGM_NONFINITE if the resident state holds a NaN or Inf */
+int dgm_check_state(dgm_ctx* ctx);
+/* time and step counters of the resident run */
+int dgm_device_clock(dgm_ctx* ctx, double* time, int32_t* steps, double* last_dt);
+/* Lift coefficient Cl = (Fy cos a - Fx sin a) / (0.5 rho V^2 c) of the wall pressure (extension), chord c. */
+int dgm_lift_coefficient(dgm_ctx* ctx, double chord, double* cl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,187 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle on the same seeded inputs.
+
+Bar: FP64 mode within 1e-12 (max-norm relative per conserved variable); FP32-bulk mode within
+the tolerances stated per test. Troubled-cell sets must match exactly.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from dgm_in_rust_amd import Context, SpatialDisc, TemperalDisc, meshgen
+from dgm_in_rust_amd._lib import DgmError
+from oracle.refcpu import RefCase
+from tests.helpers import flow_state, rel_err, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+LIM = {"off": 0, "sequential": 1, "jacobi": 2}
+TOL64 = 1e-12
+
+
+def pair(mesh, order, limiter="sequential", precision="fp64", **kw):
+    ref = RefCase(mesh, order=order, limiter=LIM[limiter], **kw)
+    ctx = Context(mesh, order=order, limiter=limiter, precision=precision, **kw)
+    return ref, ctx
+
+
+def meshes(airfoil):
+    return {
+        "airfoil_p2": (airfoil, 2, dict(mach=0.8, aoa_deg=1.5)),
+        "ogrid_p3": (meshgen.naca0012_ogrid(40, 10, seed=3), 3, dict(mach=0.8, aoa_deg=1.25)),
+        "ogrid_p1": (meshgen.naca0012_ogrid(32, 8, seed=4), 1, dict(mach=0.5, aoa_deg=1.25)),
+        "periodic_p4": (meshgen.periodic_square(6, seed=5), 4, dict(mach=0.5, aoa_deg=0.0)),
+        "box_p2": (meshgen.box_farfield(5, seed=6), 2, dict(mach=0.5, aoa_deg=1.25)),
+    }
+
+
+@pytest.fixture(scope="module")
+def all_meshes(airfoil_mesh):
+    return meshes(airfoil_mesh)
+
+
+@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3", "ogrid_p1", "periodic_p4", "box_p2"])
+def test_residual_fp64(all_meshes, name):
+    mesh, order, kw = all_meshes[name]
+    ref, ctx = pair(mesh, order, **kw)
+    U = flow_state(ref, seed=11, shock=True, mach=kw["mach"])
+    Rref, s, _ = ref.compute_residuals(U)
+    assert s == 0
+    R = SpatialDisc(ctx).residuals(U)
+    assert rel_err(R, Rref) <= TOL64, rel_err(R, Rref)
+
+
+def test_residual_accumulate_semantics(all_meshes):
+    mesh, order, kw = all_meshes["airfoil_p2"]
+    ref, ctx = pair(mesh, order, **kw)
+    U = flow_state(ref, seed=12, mach=0.8)
+    R0 = np.random.default_rng(1).standard_normal(ref.shape) * 1e3
+    Rref, s, _ = ref.compute_residuals(U, R0)
+    R = R0.copy()
+    SpatialDisc(ctx).compute_residuals(R, U)
+    assert rel_err(R, Rref) <= TOL64
+
+
+@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3", "periodic_p4"])
+def test_time_step(all_meshes, name):
+    mesh, order, kw = all_meshes[name]
+    ref, ctx = pair(mesh, order, **kw)
+    U = flow_state(ref, seed=13, shock=True, mach=kw["mach"])
+    dref = ref.compute_time_step(U)
+    d = TemperalDisc(ctx).compute_time_step(U)
+    assert abs(d - dref) <= 1e-14 * abs(dref)
+
+
+@pytest.mark.parametrize("limiter", ["sequential", "jacobi"])
+@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3", "ogrid_p1", "periodic_p4"])
+def test_limiter_fp64(all_meshes, name, limiter):
+    mesh, order, kw = all_meshes[name]
+    ref, ctx = pair(mesh, order, limiter=limiter, **kw)
+    U = flow_state(ref, seed=14, shock=True, mach=kw["mach"], noise=0.02)
+    Uref, tref, s = ref.apply_limiter(U)
+    assert s == 0
+    assert tref.sum() > 0, "state must trouble some cells"
+    V = U.copy()
+    t = np.zeros(ctx.ne, np.uint8)
+    SpatialDisc(ctx).apply_limiter(V, t)
+    assert np.array_equal(t.astype(bool), tref)
+    assert rel_err(V, Uref) <= TOL64, rel_err(V, Uref)
+
+
+@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3"])
+def test_rk3_step_fp64(all_meshes, name):
+    mesh, order, kw = all_meshes[name]
+    ref, ctx = pair(mesh, order, **kw)
+    U = flow_state(ref, seed=15, shock=True, mach=kw["mach"], amp=0.02)
+    dt = ref.compute_time_step(U)
+    Uref, s = ref.rk3(U, dt)
+    assert s == 0
+    V = U.copy()
+    TemperalDisc(ctx).rk3(SpatialDisc(ctx), ctx.zeros(), V, dt)
+    assert rel_err(V, Uref) <= TOL64, rel_err(V, Uref)
+
+
+def test_time_march_reference_config(all_meshes):
+    """airfoil.msh, p = 2, (6, 3) rules, HLLC, sequential WENO, reference CFL formula, 10 steps."""
+    mesh, order, kw = all_meshes["airfoil_p2"]
+    ref, ctx = pair(mesh, order, **kw)
+    U0 = ref.initial_solution(1.0, 1.5, 0.8, 1.0e5)
+    Uref, s, n, t = ref.time_march(U0, 10, 5.0)
+    assert s == 0 and n == 10
+    V = U0.copy()
+    td = TemperalDisc(ctx, final_step=10, final_time=5.0)
+    st = td.time_march(SpatialDisc(ctx), ctx.zeros(), V)
+    assert st["steps"] == 10
+    assert abs(st["time"] - t) <= 1e-14 * t
+    assert rel_err(V, Uref) <= TOL64, rel_err(V, Uref)
+
+
+def test_time_march_ogrid_p3_jacobi(all_meshes):
+    mesh, order, kw = all_meshes["ogrid_p3"]
+    ref, ctx = pair(mesh, order, limiter="jacobi", cfl=0.3, **kw)
+    U0 = ref.initial_solution(1.0, kw["aoa_deg"], kw["mach"], 1.0e5)
+    Uref, s, n, t = ref.time_march(U0, 8, 100.0)
+    assert s == 0
+    V = U0.copy()
+    st = TemperalDisc(ctx, final_step=8, final_time=100.0).time_march(SpatialDisc(ctx), ctx.zeros(), V)
+    assert st["steps"] == n
+    assert rel_err(V, Uref) <= TOL64, rel_err(V, Uref)
+
+
+def test_final_time_clamp(all_meshes):
+    mesh, order, kw = all_meshes["box_p2"]
+    ref, ctx = pair(mesh, order, **kw)
+    U0 = flow_state(ref, seed=16, mach=0.5)
+    dt = ref.compute_time_step(U0)
+    T = 2.5 * dt
+    Uref, s, n, t = ref.time_march(U0, 100, T)
+    V = U0.copy()
+    st = TemperalDisc(ctx, final_step=100, final_time=T).time_march(SpatialDisc(ctx), ctx.zeros(), V)
+    assert st["steps"] == n and abs(st["time"] - t) <= 1e-15 * T
+    assert rel_err(V, Uref) <= TOL64
+
+
+@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3"])
+def test_residual_fp32_bulk(all_meshes, name):
+    mesh, order, kw = all_meshes[name]
+    ref, ctx = pair(mesh, order, precision="fp32", **kw)
+    U = flow_state(ref, seed=17, shock=True, mach=kw["mach"])
+    Rref, s, _ = ref.compute_residuals(U)
+    R = SpatialDisc(ctx).residuals(U)
+    # FP32 arithmetic on O(1e5) energies: compare against the residual scale
+    assert rel_l2(R, Rref) <= 2e-5, rel_l2(R, Rref)
+
+
+def test_time_march_fp32_bulk(all_meshes):
+    mesh, order, kw = all_meshes["ogrid_p3"]
+    ref, ctx = pair(mesh, order, limiter="off", precision="fp32", cfl=0.3, **kw)
+    U0 = flow_state(ref, seed=18, mach=kw["mach"], amp=0.02)
+    Uref, s, n, t = ref.time_march(U0, 20, 100.0)
+    V = U0.copy()
+    TemperalDisc(ctx, final_step=20, final_time=100.0).time_march(SpatialDisc(ctx), ctx.zeros(), V)
+    assert rel_l2(V, Uref) <= 1e-6, rel_l2(V, Uref)
+
+
+def test_negative_density_reports_edge(all_meshes):
+    mesh, order, kw = all_meshes["airfoil_p2"]
+    ref, ctx = pair(mesh, order, **kw)
+    U = flow_state(ref, seed=19, mach=0.8)
+    U[1234, 0, 0] = -0.5  # negative mean density in one interior element
+    _, s, bad = ref.compute_residuals(U)
+    assert s == 1
+    with pytest.raises(DgmError) as ei:
+        SpatialDisc(ctx).residuals(U)
+    assert ei.value.status == 1
+    assert ei.value.edge == bad
+
+
+def test_negative_pressure_status(all_meshes):
+    mesh, order, kw = all_meshes["ogrid_p3"]
+    ref, ctx = pair(mesh, order, **kw)
+    U = flow_state(ref, seed=20, mach=0.8)
+    U[77, 3, 0] = 0.0  # zero energy: negative pressure on its edges
+    _, s, bad = ref.compute_residuals(U)
+    assert s == 2
+    with pytest.raises(DgmError) as ei:
+        SpatialDisc(ctx).residuals(U)
+    assert ei.value.status == 2 and ei.value.edge == bad
