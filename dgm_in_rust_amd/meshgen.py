@@ -289,7 +289,7 @@ def write_gmsh(mesh: MeshInput, path: str) -> None:
         for i in range(nv):
             f.write(f"{i + 1}\n")
         for x, y in mesh.vxy:
-            f.write(f"{x!r} {y!r} 0\n")
+            f.write(f"{float(x)!r} {float(y)!r} 0\n")
         f.write("$EndNodes\n")
         ne = mesh.n_elements
         nl = int(mesh.eb_count.sum())

@@ -1,0 +1,169 @@
+"""Mesh ingest and connectivity (CPU).
+
+Pins: the reference's own output outputfiles/mesh.vtk (points and element -> vertex order after
+GmshParser + process_mesh, src/io/write_results.rs:9-24) for airfoil.msh; the numbering rules of
+src/initialization.rs:112-261 checked as invariants; product connectivity (libdgm.so host-side
+set-up) bit-exact against the oracle on the reference mesh and on generated meshes.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from dgm_in_rust_amd import Context, meshgen
+from oracle import refcpu
+from oracle.refcpu import RefCase
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_gmsh_parse_matches_reference_vtk(airfoil_mesh):
+    vtk = np.load(os.path.join(GOLD, "airfoil_vtk.npz"))
+    m = refcpu.gmsh_parse(os.path.join(GOLD, "airfoil.msh"))
+    assert np.array_equal(m.vxy, vtk["points"][:, :2])
+    ref = RefCase(m, order=2)
+    el = ref.elements()
+    assert np.array_equal(el["ivertices"], vtk["cells"])
+    # the product's reader and set-up give the same element -> vertex order
+    assert np.array_equal(airfoil_mesh.vxy, vtk["points"][:, :2])
+    ctx = Context(airfoil_mesh, order=2, host_only=True)
+    assert np.array_equal(ctx.connectivity()["element_vertices"], vtk["cells"])
+
+
+def test_airfoil_counts(airfoil_mesh):
+    ref = RefCase(airfoil_mesh, order=2)
+    assert (ref.ne, ref.nedge, ref.n_internal_edges) == (3738, 5699, 5515)
+    assert (ref.n_internal_elements, ref.n_boundary_elements) == (3554, 184)
+    assert list(airfoil_mesh.eb_type) == [0, 0, 0, 1, 1, 1, 1]
+    assert int(airfoil_mesh.eb_count[:3].sum()) == 104 and int(airfoil_mesh.eb_count[3:].sum()) == 80
+
+
+def numbering_invariants(ref):
+    E, L = ref.edges(), ref.lists()
+    el = ref.elements()
+    nb = ref.nedge - ref.n_internal_edges
+    # boundary edges first (ids 0..nb-1), right slot -1; interior edges after, in increasing id order
+    assert np.all(E["ielements"][:nb, 1] == -1) and np.all(E["ipatch"][:nb] >= 0)
+    assert np.array_equal(L["internal_edges"], np.arange(nb, ref.nedge))
+    # interior edge ids appear in first-visit order of the element scan
+    first = {}
+    for e in range(ref.ne):
+        for i in range(3):
+            first.setdefault(int(el["iedges"][e, i]), e * 3 + i)
+    ids = sorted(i for i in first if i >= nb)
+    assert [first[i] for i in ids] == sorted(first[i] for i in ids)
+    # left element runs ivertices[0] -> ivertices[1]; right runs it backwards
+    iv, ie, ic = el["ivertices"], E["ielements"], E["in_cell_indices"]
+    for k in range(nb, ref.nedge):
+        l, li = ie[k, 0], ic[k, 0]
+        r, ri = ie[k, 1], ic[k, 1]
+        assert iv[l, li] == E["ivertices"][k, 0] and iv[l, (li + 1) % 3] == E["ivertices"][k, 1]
+        assert iv[r, ri] == E["ivertices"][k, 1] and iv[r, (ri + 1) % 3] == E["ivertices"][k, 0]
+    # neighbours are symmetric; boundary elements are exactly those with a boundary edge
+    nbr = el["ineighbours"]
+    for e in range(ref.ne):
+        for i in range(3):
+            n = nbr[e, i]
+            if n >= 0:
+                assert e in nbr[n]
+    has_b = np.any(el["iedges"] < nb, axis=1)
+    assert np.array_equal(np.flatnonzero(has_b), L["boundary_elements"])
+    assert np.array_equal(np.flatnonzero(~has_b), L["internal_elements"])
+
+
+def test_numbering_rules_airfoil(airfoil_mesh):
+    numbering_invariants(RefCase(airfoil_mesh, order=2))
+
+
+def test_numbering_rules_ogrid():
+    numbering_invariants(RefCase(meshgen.naca0012_ogrid(24, 6, seed=2), order=2))
+
+
+def compare_connectivity(mesh, order=2):
+    ref = RefCase(mesh, order=order)
+    ctx = Context(mesh, order=order, host_only=True)
+    E, el, L = ref.edges(), ref.elements(), ref.lists()
+    c = ctx.connectivity()
+    for a, b in (("ielements", "ielements"), ("ivertices", "ivertices"), ("in_cell_indices", "in_cell_indices"),
+                 ("ipatch", "ipatch")):
+        assert np.array_equal(c[a], E[b]), a
+    assert np.array_equal(c["element_vertices"], el["ivertices"])
+    assert np.array_equal(c["element_edges"], el["iedges"])
+    assert np.array_equal(c["ineighbours"], el["ineighbours"])
+    for k in ("internal_edges", "internal_elements", "boundary_elements"):
+        assert np.array_equal(c[k], L[k]), k
+    g = ctx.geometry()
+    assert np.array_equal(g["normal"], E["normal"])
+    assert np.array_equal(g["edge_length"], E["jacob_det"])
+    assert np.array_equal(g["jacob_det"], el["jacob_det"])
+    assert np.array_equal(g["circumradius"], el["circumradius"])
+    assert np.array_equal(g["minimum_height"], el["minimum_height"])
+
+
+def test_product_connectivity_airfoil(airfoil_mesh):
+    compare_connectivity(airfoil_mesh)
+
+
+@pytest.mark.parametrize("mesh_fn", [lambda: meshgen.naca0012_ogrid(40, 12, seed=7),
+                                     lambda: meshgen.periodic_square(7, seed=8),
+                                     lambda: meshgen.box_farfield(6, seed=9)])
+def test_product_connectivity_generated(mesh_fn):
+    compare_connectivity(mesh_fn())
+
+
+def test_ogrid_sizes():
+    m1 = meshgen.naca0012_ogrid(160, 50, seed=0)
+    assert m1.n_elements == 16000
+    ref = RefCase(m1, order=2)
+    assert ref.nedge == 24160 and ref.nedge - ref.n_internal_edges == 320
+    m3 = meshgen.naca0012_ogrid(640, 100, seed=0)
+    assert m3.n_elements == 128000
+    # first layer thickness of the 50-layer grid: r_far (g - 1) / (g^n - 1) = 0.0349 chords
+    r = np.hypot(m1.vxy[160:320, 0] - m1.vxy[:160, 0], m1.vxy[160:320, 1] - m1.vxy[:160, 1])
+    assert 0.025 < np.median(r) < 0.045
+
+
+def test_gmsh_roundtrip(tmp_path):
+    m = meshgen.naca0012_ogrid(20, 5, seed=1)
+    p = str(tmp_path / "og.msh")
+    meshgen.write_gmsh(m, p)
+    a = refcpu.gmsh_parse(p)     # the oracle's GmshParser restatement
+    b = meshgen.read_gmsh(p)     # the product reader
+    for x in (a, b):
+        assert np.array_equal(x.vxy, m.vxy)
+        assert np.array_equal(x.el_nodes, m.el_nodes)
+        assert np.array_equal(x.eb_nodes, m.eb_nodes)
+        assert np.array_equal(x.eb_type, m.eb_type) and np.array_equal(x.eb_count, m.eb_count)
+
+
+def test_level_schedule_is_exact_order(airfoil_mesh):
+    """Levels of the element-order sweep: adjacent elements never share a level and lower-numbered
+    neighbours always come first (so the in-place sweep's reads are reproduced)."""
+    ctx = Context(airfoil_mesh, order=2, host_only=True)
+    assert ctx.info.n_limiter_levels == 12  # measured on airfoil.msh
+    nbr = ctx.connectivity()["ineighbours"]
+    lvl = np.zeros(ctx.ne, np.int64)
+    for e in range(ctx.ne):
+        lo = [lvl[n] + 1 for n in nbr[e] if 0 <= n < e]
+        lvl[e] = max(lo) if lo else 0
+    for e in range(ctx.ne):
+        for n in nbr[e]:
+            if n >= 0:
+                assert lvl[n] != lvl[e]
+                assert (lvl[n] < lvl[e]) == (n < e)
+
+
+def test_bad_meshes_rejected():
+    from dgm_in_rust_amd._lib import DgmError
+    m = meshgen.box_farfield(3, seed=0)
+    cw = m.contiguous()
+    cw.el_nodes = cw.el_nodes[:, ::-1].copy()
+    with pytest.raises(DgmError) as e:
+        Context(cw, order=2, host_only=True)
+    assert e.value.status == 5
+    with pytest.raises(DgmError) as e:
+        Context(m, order=5, host_only=True)
+    assert e.value.status == 4
+    with pytest.raises(DgmError) as e:
+        Context(m, order=2, n_cell_gp=7, host_only=True)
+    assert e.value.status == 4

@@ -1,0 +1,223 @@
+"""Pins of the C++ oracle (CPU): closed forms and invariants of the fluxes, eigenmatrix, RK3 and
+residual, plus agreement with the independent numpy restatement (oracle/numpy_ref.py) on small
+meshes for every quirk-bearing operation (residual, KXRCF/WENO in both sweep orders, time step,
+RK3 steps)."""
+import math
+
+import numpy as np
+import pytest
+
+from dgm_in_rust_amd import meshgen
+from oracle import refcpu
+from oracle.numpy_ref import NumpyRef
+from oracle.refcpu import RefCase
+from tests.helpers import flow_state, prim_to_cons, rel_err
+
+G = 1.4
+
+
+def rand_state(rng, n=1, rho=(0.5, 2.0), vel=300.0, p=(0.5e5, 2e5)):
+    r = rng.uniform(*rho, n)
+    u, v = rng.uniform(-vel, vel, n), rng.uniform(-vel, vel, n)
+    pp = rng.uniform(*p, n)
+    return prim_to_cons(r, u, v, pp)
+
+
+# ---- flux::flux / flux::hllc -----------------------------------------------------------------
+
+def test_euler_flux_closed_form():
+    q = prim_to_cons(np.array([1.3]), np.array([120.0]), np.array([-40.0]), np.array([9e4]))[0]
+    f, g = refcpu.flux(q)
+    r, u, v, p = 1.3, 120.0, -40.0, 9e4
+    E = q[3]
+    assert np.allclose(f, [r * u, r * u * u + p, r * u * v, u * (E + p)], rtol=1e-14)
+    assert np.allclose(g, [r * v, r * u * v, r * v * v + p, v * (E + p)], rtol=1e-14)
+
+
+def test_hllc_consistency_and_rotation():
+    rng = np.random.default_rng(0)
+    for q in rand_state(rng, 50):
+        th = rng.uniform(0, 2 * np.pi)
+        n = np.array([math.cos(th), math.sin(th)])
+        F, s = refcpu.hllc(q, q, n)
+        f, g = refcpu.flux(q)
+        assert s == 0
+        assert np.allclose(F, f * n[0] + g * n[1], rtol=1e-12, atol=1e-9 * np.abs(q).max())
+
+
+def test_hllc_supersonic_is_upwind():
+    rng = np.random.default_rng(1)
+    c = math.sqrt(G * 1e5 / 1.0)
+    for _ in range(20):
+        ql = prim_to_cons(np.array([1.0]), np.array([2.5 * c]), np.array([rng.uniform(-50, 50)]), np.array([1e5]))[0]
+        qr = prim_to_cons(np.array([1.1]), np.array([2.4 * c]), np.array([rng.uniform(-50, 50)]), np.array([1.1e5]))[0]
+        F, s = refcpu.hllc(ql, qr, np.array([1.0, 0.0]))
+        f, _ = refcpu.flux(ql)
+        assert s == 0 and np.allclose(F, f, rtol=1e-14)
+
+
+def test_hllc_status_codes():
+    q = prim_to_cons(np.array([1.0]), np.array([10.0]), np.array([0.0]), np.array([1e5]))[0]
+    bad = q.copy()
+    bad[0] = -1.0
+    assert refcpu.hllc(bad, q, np.array([1.0, 0.0]))[1] == 1
+    neg_p = q.copy()
+    neg_p[3] = 0.1
+    assert refcpu.hllc(q, neg_p, np.array([1.0, 0.0]))[1] == 2
+
+
+def test_hllc_orientation_dependence_exists():
+    """Quirk Q18: with the altered p* the flux is not antisymmetric under (L,R,n) -> (R,L,-n) for some
+    strongly compressive pairs; the GPU therefore evaluates each edge once, in the reference's orientation."""
+    rng = np.random.default_rng(2)
+    asym = 0
+    for _ in range(2000):
+        ql, qr = rand_state(rng, 2, vel=900.0)
+        n = np.array([1.0, 0.0])
+        F1, s1 = refcpu.hllc(ql, qr, n)
+        F2, s2 = refcpu.hllc(qr, ql, -n)
+        if s1 == 0 and s2 == 0 and np.all(np.isfinite(F1)) and np.all(np.isfinite(F2)):
+            if not np.allclose(F1, -F2, rtol=1e-9, atol=1e-6):
+                asym += 1
+    assert asym > 0
+
+
+def test_eigenmatrix_inverse_and_diagonalisation():
+    rng = np.random.default_rng(3)
+    for q in rand_state(rng, 20):
+        th = rng.uniform(0, 2 * np.pi)
+        nx, ny = math.cos(th), math.sin(th)
+        L, R = refcpu.eigenmatrix(q, nx, ny)
+        assert np.allclose(L @ R, np.eye(4), atol=1e-12)
+        # L A_n R = diag(vn - c, vn, vn, vn + c), A_n = d(F nx + G ny)/dq by central differences
+        A = np.zeros((4, 4))
+        for k in range(4):
+            h = 1e-6 * abs(q[k]) + 1e-8
+            dq = np.zeros(4)
+            dq[k] = h
+            fp, gp = refcpu.flux(q + dq)
+            fm, gm = refcpu.flux(q - dq)
+            A[:, k] = ((fp - fm) * nx + (gp - gm) * ny) / (2 * h)
+        r, u, v = q[0], q[1] / q[0], q[2] / q[0]
+        p = (G - 1) * (q[3] - 0.5 * r * (u * u + v * v))
+        c, vn = math.sqrt(G * p / r), u * nx + v * ny
+        D = L @ A @ R
+        assert np.allclose(np.diag(D), [vn - c, vn, vn, vn + c], rtol=1e-5, atol=1e-3)
+        assert np.abs(D - np.diag(np.diag(D))).max() <= 1e-4 * np.abs(D).max()
+
+
+# ---- RK3 -------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("z", [-0.1, -0.5, -1.2, 0.3])
+def test_rk3_stability_polynomial(z):
+    u0 = np.array([1.0, -2.0, 3.5])
+    lam, dt = z / 0.01, 0.01
+    got = refcpu.rk3_linear(u0, lam, dt)
+    assert np.allclose(got, u0 * (1 + z + z * z / 2 + z ** 3 / 6), rtol=1e-14)
+
+
+# ---- residual invariants ---------------------------------------------------------------------
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_freestream_preserved_periodic(order):
+    ref = RefCase(meshgen.periodic_square(5, seed=1), order=order, limiter=0, mach=0.5, aoa_deg=30.0)
+    U = ref.initial_solution(1.0, 30.0, 0.5, 1e5)
+    R, s, _ = ref.compute_residuals(U)
+    assert s == 0
+    assert np.abs(R).max() <= 1e-9 * np.abs(U).max() / 0.1  # round-off of O(1e5) fluxes over h ~ 2
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_freestream_preserved_farfield_box(order):
+    ref = RefCase(meshgen.box_farfield(4, seed=2), order=order, limiter=0, mach=0.5, aoa_deg=10.0)
+    U = ref.initial_solution(1.0, 10.0, 0.5, 1e5)
+    R, s, _ = ref.compute_residuals(U)
+    assert s == 0 and np.abs(R).max() <= 1e-8 * np.abs(U).max() * 4
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_discrete_conservation_periodic(order):
+    """sum_e |e| m_0 R[e,:,0] = 0: every interior flux enters its two elements with opposite signs."""
+    ref = RefCase(meshgen.periodic_square(6, seed=3), order=order, limiter=0)
+    U = flow_state(ref, seed=4, shock=True, mach=0.5)
+    R, s, _ = ref.compute_residuals(U)
+    J = ref.elements()["jacob_det"]
+    tot = (0.5 * J[:, None] * R[:, :, 0]).sum(axis=0)
+    scale = (0.5 * J[:, None] * np.abs(R[:, :, 0])).sum(axis=0)
+    assert np.all(np.abs(tot) <= 1e-13 * scale)
+
+
+# ---- second restatement ----------------------------------------------------------------------
+
+LIMS = {"sequential": 1, "jacobi": 2}
+
+
+def both(mesh, order, limiter="sequential", cfl=0.0, mach=0.8, aoa=1.5):
+    ref = RefCase(mesh, order=order, limiter=LIMS.get(limiter, 0), cfl=cfl, mach=mach, aoa_deg=aoa)
+    nref = NumpyRef(mesh, ref.tables(), order=order, cfl=cfl, fs=(mach, aoa, 1.0, 1e5, 1.4), limiter=limiter)
+    return ref, nref
+
+
+SMALL = {
+    "box_p2": (lambda: meshgen.box_farfield(3, seed=5), 2),
+    "ogrid_p2": (lambda: meshgen.naca0012_ogrid(12, 3, seed=6), 2),
+    "ogrid_p1": (lambda: meshgen.naca0012_ogrid(12, 3, seed=7), 1),
+    "ogrid_p3": (lambda: meshgen.naca0012_ogrid(10, 2, seed=8), 3),
+    "periodic_p4": (lambda: meshgen.periodic_square(4, seed=9), 4),
+}
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_numpy_restatement_residual(name):
+    fn, order = SMALL[name]
+    ref, nref = both(fn(), order)
+    for seed in (0, 1):
+        U = flow_state(ref, seed=seed, shock=True, mach=0.5)
+        R, s, _ = ref.compute_residuals(U)
+        assert s == 0
+        assert rel_err(nref.residual(U), R) <= 1e-12
+
+
+@pytest.mark.parametrize("limiter", ["sequential", "jacobi"])
+@pytest.mark.parametrize("name", list(SMALL))
+def test_numpy_restatement_limiter(name, limiter):
+    fn, order = SMALL[name]
+    ref, nref = both(fn(), order, limiter)
+    hits = 0
+    for seed in range(4):
+        U = flow_state(ref, seed=seed, shock=True, mach=0.5, noise=0.03)
+        Ua, ta, s = ref.apply_limiter(U)
+        Ub, tb = nref.apply_limiter(U)
+        assert s == 0
+        assert np.array_equal(ta, tb)
+        assert rel_err(Ub, Ua) <= 1e-12
+        hits += int(ta.sum())
+    assert hits > 0
+
+
+def test_numpy_restatement_kxrcf_quirks():
+    """Cells without an inflow face give 0/0 = NaN -> not troubled (Q8)."""
+    ref, nref = both(meshgen.box_farfield(3, seed=5), 2)
+    U = ref.initial_solution(1.0, 0.0, 0.0, 1e5)   # fluid at rest: no inflow anywhere
+    for e in range(ref.ne):
+        assert ref.kxrcf(U, e) is False and nref.kxrcf(U, e) is False
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_numpy_restatement_time_step(name):
+    fn, order = SMALL[name]
+    ref, nref = both(fn(), order)
+    U = flow_state(ref, seed=3, shock=True, mach=0.5)
+    assert abs(ref.compute_time_step(U) - nref.time_step(U)) <= 1e-15 * nref.time_step(U)
+
+
+@pytest.mark.parametrize("name,limiter", [("ogrid_p2", "sequential"), ("ogrid_p2", "jacobi"), ("box_p2", "sequential"),
+                                          ("ogrid_p3", "jacobi"), ("periodic_p4", "sequential")])
+def test_numpy_restatement_time_march(name, limiter):
+    fn, order = SMALL[name]
+    ref, nref = both(fn(), order, limiter, mach=0.5)
+    U0 = flow_state(ref, seed=5, shock=True, mach=0.5, amp=0.02)
+    Ua, s, n, t = ref.time_march(U0, 3, 10.0)
+    Ub, nb, tb = nref.time_march(U0, 3, 10.0)
+    assert s == 0 and n == nb == 3 and abs(t - tb) <= 1e-15 * t
+    assert rel_err(Ub, Ua) <= 1e-12
