@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Throughput of the DG-Euler explicit SSP-RK3 step on one MI355X, in GDOF * RK-stage / s.
+
+Workload (the transonic single case at scale): NACA 0012 O-grid, 640 surface points x 100 layers
+= 128 000 triangles, Dubiner p = 3 (10 modes, 16-point volume and 5-point edge rules), HLLC,
+WENO limiter on (Jacobi sweep), M = 0.8, AoA = 1.25 deg, freestream start, CFL from the
+reference formula. A "step" is one SSP-RK3 step: time-step reduction + 3 x (surface flux,
+element update, KXRCF, WENO, apply), replayed from a captured HIP graph with the state
+resident in HBM. The headline value is the FP32-bulk path (FP64 u^n, RK accumulation, WENO
+weights and positivity tests); the plain FP64 path is measured in the same run.
+
+GDOF * stage / s = n_elements * 4 * n_basis * 3 * steps / seconds / 1e9.
+
+The roofline object is for the dominant kernel, with its algorithmic bytes per launch and its
+mean launch time from HIP events on the context stream (dgm_profile_steps); traffic, when given
+(--traffic FETCH_KIB,WRITE_KIB from rocprofv3 counter runs), is reported as measured.
+The CPU baseline times the oracle (FP64 C++ restatement of the reference, OpenMP in the three
+residual folds like the reference's rayon folds) on a few steps of the same case.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "GDOF·RK-stage/s at p=3 (FP32-bulk vs FP64) + p50/p99 case latency"
+
+
+def kernel_bytes(np_, ne_pts, tsize, nel, nedge):
+    """Algorithmic bytes per launch: every array the kernel must read or write, counted once.
+
+    element (per element): stage input 4Np*T, its three edges' point fluxes 3*Ne*4*T, metric and 2/J
+    5*T, edge ids 3*4, and the RK stage combination averaged over the three stages
+    (FP64: write; read u^n + write; read u^n + write u^n.  FP32-bulk: write d, s; read d, s^n + write
+    d, s; read d, u^n(8) + write u^n(8), s^n).
+    surface (per edge): both neighbours' modes 2*4Np*T, two element ids + info word 12, normal and
+    half length 3*T, and the point fluxes Ne*4*T written.
+    """
+    n = 4 * np_
+    if tsize == 8:
+        rk = (n * 8 + 2 * n * 8 + 2 * n * 8) / 3.0
+    else:
+        rk = (2 * n * 4 + 4 * n * 4 + (n * 4 + n * 8 + n * 8 + n * 4)) / 3.0
+    element = nel * (n * tsize + 3 * ne_pts * 4 * tsize + 5 * tsize + 12 + rk)
+    surface = nedge * (2 * n * tsize + 12 + 3 * tsize + ne_pts * 4 * tsize)
+    return {"element": element, "surface": surface}
+
+
+def stage_bytes_survey(np_, fp32):
+    """Per element-stage algorithmic bytes of the survey's accounting: B64 = 117.33 Np + 91, B32 = 80 Np + 53."""
+    return (80.0 * np_ + 53.0) if fp32 else (117.33 * np_ + 91.0)
+
+
+def run_gpu(mesh, precision, steps, warmup, order, mach, aoa):
+    from dgm_in_rust_amd import Context
+    ctx = Context(mesh, order=order, precision=precision, limiter="jacobi", mach=mach, aoa_deg=aoa)
+    U0 = ctx.initial_solution(1.0, aoa, mach, 1.0e5)
+    ctx.upload(U0)
+    if warmup:
+        ctx.steps_async(warmup)
+        ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.steps_async(steps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    tnow, nsteps, dt = ctx.clock()
+    ctx.check_state()
+    kt = ctx.profile_steps(20)
+    info = ctx.info
+    out = dict(ctx=ctx, wall=wall, steps_done=nsteps, t=tnow, dt=dt, kt=kt, ne=info.n_elements, nedge=info.n_edges,
+               np=info.n_basis, nq=info.n_cell_gp, ng=info.n_edge_gp, cfl=info.cfl)
+    return out
+
+
+def summarise(r, steps, fp32):
+    dof = r["ne"] * 4 * r["np"]
+    value = dof * 3 * steps / r["wall"] / 1e9
+    tsize = 4 if fp32 else 8
+    kb = kernel_bytes(r["np"], r["ng"], tsize, r["ne"], r["nedge"])
+    kt = r["kt"]
+    per = {"element": kt["element_ms"], "surface": kt["surface_ms"], "detect": kt["detect_ms"],
+           "rebuild": kt["rebuild_ms"], "apply": kt["apply_ms"], "time_step": kt["time_step_ms"]}
+    dom = max(("element", "surface"), key=lambda k: per[k])
+    ach = kb[dom] / (per[dom] * 1e-3) / 1e9
+    stage_ms = 1e3 * r["wall"] / steps / 3.0
+    sb = stage_bytes_survey(r["np"], fp32) * r["ne"]
+    return dict(value=value, ms_per_step=1e3 * r["wall"] / steps, dominant=dom, kernel_ms=per,
+                roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                              traffic=None, kernel=("k_element" if dom == "element" else "k_surface"),
+                              bytes_per_launch=kb[dom]),
+                stage=dict(bytes_per_element_stage=stage_bytes_survey(r["np"], fp32), ms=stage_ms,
+                           achieved=sb / (stage_ms * 1e-3) / 1e9, frac=sb / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+
+
+def cpu_baseline(mesh, order, mach, aoa, nsteps):
+    from oracle.refcpu import RefCase
+    threads = max(1, min(16, os.cpu_count() or 1))
+    ref = RefCase(mesh, order=order, limiter=1, mach=mach, aoa_deg=aoa, nthreads=threads)
+    U = ref.initial_solution(1.0, aoa, mach, 1.0e5)
+    t0 = time.perf_counter()
+    U, s, n, t = ref.time_march(U, nsteps, 1.0e9)
+    sec = time.perf_counter() - t0
+    val = ref.ne * 4 * ref.nbasis * 3 * n / sec / 1e9
+    return dict(value=val, unit="GDOF·stage/s", cores=threads, kind="port",
+                sample=f"{n} SSP-RK3 steps of the same 128k-triangle p=3 case from freestream (oracle, FP64, "
+                       f"sequential WENO; {threads} OpenMP threads in the residual folds, {sec:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--surface", type=int, default=640)
+    ap.add_argument("--layers", type=int, default=100)
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--traffic", default=None, help="FETCH_KIB,WRITE_KIB per launch of the dominant FP32 kernel")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+    from dgm_in_rust_amd import meshgen
+    mach, aoa = 0.8, 1.25
+    mesh = meshgen.naca0012_ogrid(args.surface, args.layers, seed=0)
+    r32 = run_gpu(mesh, "fp32", args.steps, args.warmup, args.order, mach, aoa)
+    s32 = summarise(r32, args.steps, True)
+    print(f"fp32-bulk: {s32['value']:.2f} GDOF*stage/s, {s32['ms_per_step']:.4f} ms/step, kernels {s32['kernel_ms']}",
+          file=sys.stderr, flush=True)
+    s64 = None
+    if not args.no_fp64:
+        r64 = run_gpu(mesh, "fp64", args.steps, args.warmup, args.order, mach, aoa)
+        s64 = summarise(r64, args.steps, False)
+        print(f"fp64: {s64['value']:.2f} GDOF*stage/s, {s64['ms_per_step']:.4f} ms/step, kernels {s64['kernel_ms']}",
+              file=sys.stderr, flush=True)
+    if args.traffic:
+        f, w = [float(x) for x in args.traffic.split(",")]
+        s32["roofline"]["traffic"] = (f + w) * 1024.0
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mesh, args.order, mach, aoa, args.cpu_steps)
+        print(f"cpu baseline: {cpu['value']:.5f} GDOF*stage/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+    line = {
+        "metric": METRIC, "value": s32["value"], "unit": "GDOF·stage/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": s32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"naca0012_ogrid_{args.surface}x{args.layers}_p{args.order}_M0.8_aoa1.25_weno",
+                   "n_elements": r32["ne"], "order": args.order, "n_cell_gp": r32["nq"], "n_edge_gp": r32["ng"],
+                   "mach": mach, "aoa_deg": aoa, "limiter": "jacobi", "cfl": r32["cfl"],
+                   "precision": "fp32_bulk (fp64 u^n, RK accumulation, WENO weights, positivity)",
+                   "parallelism": "single_gpu", "start": "freestream"},
+        "roofline": s32["roofline"],
+        "stage_roofline": s32["stage"],
+        "kernel_ms": s32["kernel_ms"],
+        "cpu_baseline": cpu,
+    }
+    if s64:
+        line["fp64"] = {"value": s64["value"], "ms_per_step": s64["ms_per_step"], "roofline": s64["roofline"],
+                        "stage_roofline": s64["stage"], "kernel_ms": s64["kernel_ms"]}
+        line["fp32_over_fp64"] = s32["value"] / s64["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

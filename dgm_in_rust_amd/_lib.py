@@ -22,7 +22,8 @@ EXPORTED = ["dgm_ctx_create", "dgm_ctx_create_host", "dgm_ctx_destroy", "dgm_ctx
             "dgm_get_geometry", "dgm_get_tables", "dgm_initial_solution", "dgm_project", "dgm_cell_points_xy",
             "dgm_compute_residuals", "dgm_apply_limiter", "dgm_compute_time_step", "dgm_rk3", "dgm_time_march",
             "dgm_state_upload", "dgm_state_download", "dgm_device_time_march", "dgm_device_steps_async",
-            "dgm_synchronize", "dgm_stream", "dgm_check_state", "dgm_device_clock", "dgm_lift_coefficient"]
+            "dgm_synchronize", "dgm_stream", "dgm_check_state", "dgm_device_clock", "dgm_lift_coefficient",
+            "dgm_profile_steps"]
 
 
 class Mesh(C.Structure):
@@ -53,6 +54,12 @@ class Error(C.Structure):
 class MarchStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("time", C.c_double), ("last_dt", C.c_double),
                 ("troubled_cells_last_stage", C.c_int64)]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("surface_ms", C.c_double), ("element_ms", C.c_double), ("detect_ms", C.c_double),
+                ("rebuild_ms", C.c_double), ("apply_ms", C.c_double), ("time_step_ms", C.c_double),
+                ("control_ms", C.c_double), ("step_ms", C.c_double), ("steps", C.c_int32)]
 
 
 class DgmError(RuntimeError):
@@ -101,6 +108,7 @@ def lib():
     L.dgm_check_state.argtypes = [vp]
     L.dgm_device_clock.argtypes = [vp, _f64p, _i32p, _f64p]
     L.dgm_lift_coefficient.argtypes = [vp, C.c_double, _f64p]
+    L.dgm_profile_steps.argtypes = [vp, C.c_int32, C.POINTER(KernelTimes)]
     _lib = L
     return L
 

@@ -655,6 +655,60 @@ int dgm_device_clock(dgm_ctx* c, double* t, int32_t* steps, double* last_dt) {
   return DGM_OK;
 }
 
+int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
+  if (!c || !c->stream || !out || n < 1) return DGM_BAD_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  const int nev = 2 + 1 + 3 * Solver::N_PARTS + 1 + 1;
+  std::vector<hipEvent_t> ev(nev);
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  double acc[Solver::N_PARTS] = {0, 0, 0, 0, 0};
+  double t_dt = 0.0, t_ctl = 0.0, t_step = 0.0;
+  for (int it = 0; it < n; ++it) {
+    int k = 0;
+    HIPCHK(c, hipEventRecord(ev[k++], c->stream));
+    c->solver->time_step(c->S, c->stream);
+    HIPCHK(c, hipEventRecord(ev[k++], c->stream));
+    launch_step_begin(c->S.ctl, c->stream);
+    HIPCHK(c, hipEventRecord(ev[k++], c->stream));
+    for (int st = 0; st < 3; ++st)
+      for (int part = 0; part < Solver::N_PARTS; ++part) {
+        c->solver->stage_part(c->S, st, part, c->stream);
+        HIPCHK(c, hipEventRecord(ev[k++], c->stream));
+      }
+    launch_step_end(c->S.ctl, c->stream);
+    HIPCHK(c, hipEventRecord(ev[k++], c->stream));
+    HIPCHK(c, hipEventSynchronize(ev[k - 1]));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    t_dt += ms;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[1], ev[2]));
+    t_ctl += ms;
+    int idx = 2;
+    for (int st = 0; st < 3; ++st)
+      for (int part = 0; part < Solver::N_PARTS; ++part, ++idx) {
+        HIPCHK(c, hipEventElapsedTime(&ms, ev[idx], ev[idx + 1]));
+        acc[part] += ms;
+      }
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[idx], ev[idx + 1]));
+    t_ctl += ms;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[idx + 1]));
+    t_step += ms;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  out->time_step_ms = t_dt / n;
+  out->control_ms = t_ctl / (2.0 * n);
+  out->surface_ms = acc[0] / (3.0 * n);
+  out->element_ms = acc[1] / (3.0 * n);
+  out->detect_ms = acc[2] / (3.0 * n);
+  out->rebuild_ms = acc[3] / (3.0 * n);
+  out->apply_ms = acc[4] / (3.0 * n);
+  out->step_ms = t_step / n;
+  out->steps = n;
+  Ctl h;
+  if (int s = read_ctl(c, h)) return s;
+  return ctl_status(c, h);
+}
+
 int dgm_lift_coefficient(dgm_ctx* c, double chord, double* cl) {
   if (!c || !c->stream || !cl || !(chord > 0.0)) return DGM_BAD_ARGUMENT;
   HIPCHK(c, hipMemsetAsync(c->d_force, 0, 2 * sizeof(double), c->stream));

@@ -1006,18 +1006,50 @@ class SolverImpl final : public Solver {
   }
 
   void stage(const DeviceState& s, int stage_idx, hipStream_t st) const override {
+    for (int part = 0; part < N_PARTS; ++part) stage_part(s, stage_idx, part, st);
+  }
+
+  // one launch (or, for the element-order sweep, one launch sequence) of a stage
+  void stage_part(const DeviceState& s, int stage_idx, int part, hipStream_t st) const override {
     StageBufs<T> b = bufs(s);
     b.in = stage_input(s, stage_idx);
-    surface(s, b.in, st, 0);
-    if (stage_idx == 0) {
-      element<OP_S0>(s, b, st, 0);
-      limiter_stage<0>(s, st, 0, 0);
-    } else if (stage_idx == 1) {
-      element<OP_S1>(s, b, st, 0);
-      limiter_stage<1>(s, st, 0, 1);
-    } else {
-      element<OP_S2>(s, b, st, 0);
-      limiter_stage<2>(s, st, 0, 2);
+    if (part == PART_SURFACE) {
+      surface(s, b.in, st, 0);
+    } else if (part == PART_ELEMENT) {
+      if (stage_idx == 0)
+        element<OP_S0>(s, b, st, 0);
+      else if (stage_idx == 1)
+        element<OP_S1>(s, b, st, 0);
+      else
+        element<OP_S2>(s, b, st, 0);
+    } else if (s.limiter != DGM_LIMITER_OFF) {
+      if (stage_idx == 0)
+        limiter_part<0>(s, st, 0, 0, part);
+      else if (stage_idx == 1)
+        limiter_part<1>(s, st, 0, 1, part);
+      else
+        limiter_part<2>(s, st, 0, 2, part);
+    }
+  }
+
+  template <int STAGE>
+  void limiter_part(const DeviceState& s, hipStream_t st, int force, int slot, int part) const {
+    StageBufs<T> b = bufs(s);
+    const T* U = limiter_input(s, STAGE);
+    LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags};
+    if (s.limiter == DGM_LIMITER_JACOBI) {
+      if (part == PART_DETECT)
+        k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
+      else if (part == PART_REBUILD)
+        k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
+      else if (part == PART_APPLY)
+        k_apply<T, NP, STAGE><<<s.apply_grid, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot, force);
+    } else if (s.limiter == DGM_LIMITER_SEQUENTIAL && part == PART_DETECT) {
+      for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
+        const int n = s.level_offsets[l + 1] - s.level_offsets[l];
+        k_limit_level<T, P, NQ, NE, STAGE><<<n, 64, 0, st>>>(s.mesh, s.levels + s.level_offsets[l], n, b, U, s.ltab,
+                                                             lb, s.ctl, slot, force);
+      }
     }
   }
 

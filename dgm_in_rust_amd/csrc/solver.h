@@ -45,6 +45,10 @@ class Solver {
   virtual void limiter(const DeviceState& s, hipStream_t st) const = 0;
   // RK stage 0, 1, 2 (surface + element + limiter), gated by Ctl::active
   virtual void stage(const DeviceState& s, int stage_idx, hipStream_t st) const = 0;
+  // the parts of a stage, in order: surface flux, element update, detector (or the whole
+  // element-order sweep), WENO rebuild, apply
+  enum { PART_SURFACE = 0, PART_ELEMENT = 1, PART_DETECT = 2, PART_REBUILD = 3, PART_APPLY = 4, N_PARTS = 5 };
+  virtual void stage_part(const DeviceState& s, int stage_idx, int part, hipStream_t st) const = 0;
   // time-step reduction into Ctl::dtmin_bits
   virtual void time_step(const DeviceState& s, hipStream_t st) const = 0;
   virtual void upload(const DeviceState& s, const double* host_layout_dev, hipStream_t st) const = 0;

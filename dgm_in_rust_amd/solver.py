@@ -158,6 +158,12 @@ class Context:
     def check_state(self):
         self.check(self._L.dgm_check_state(self.h))
 
+    def profile_steps(self, n):
+        """Run n steps eagerly with HIP events around every launch; mean ms per launch by kernel."""
+        kt = _lib.KernelTimes()
+        self.check(self._L.dgm_profile_steps(self.h, n, C.byref(kt)))
+        return {k: getattr(kt, k) for k, _ in _lib.KernelTimes._fields_}
+
     def lift_coefficient(self, chord=1.0):
         cl = C.c_double()
         self.check(self._L.dgm_lift_coefficient(self.h, chord, C.byref(cl)))

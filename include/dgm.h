@@ -157,6 +157,14 @@ void* dgm_stream(dgm_ctx* ctx);
 int dgm_check_state(dgm_ctx* ctx);
 /* time and step counters of the resident run */
 int dgm_device_clock(dgm_ctx* ctx, double* time, int32_t* steps, double* last_dt);
+/* Per-kernel timing: runs n whole steps on the resident state (eagerly, not from the graph) with HIP
+ * events around every launch on the context stream; reports the mean milliseconds per launch. */
+typedef struct {
+  double surface_ms, element_ms, detect_ms, rebuild_ms, apply_ms, time_step_ms, control_ms;
+  double step_ms;  /* mean wall time of one step between its first and last event */
+  int32_t steps;
+} dgm_kernel_times;
+int dgm_profile_steps(dgm_ctx* ctx, int32_t n_steps, dgm_kernel_times* out);
 /* Lift coefficient Cl = (Fy cos a - Fx sin a) / (0.5 rho V^2 c) of the wall pressure (extension), chord c. */
 int dgm_lift_coefficient(dgm_ctx* ctx, double chord, double* cl);
 
