@@ -68,11 +68,12 @@ def run_gpu(mesh, precision, steps, warmup, order, mach, aoa):
     ctx.synchronize()
     wall = time.perf_counter() - t0
     tnow, nsteps, dt = ctx.clock()
+    troubled = ctx.troubled_counts()[:3]
     ctx.check_state()
     kt = ctx.profile_steps(20)
     info = ctx.info
     out = dict(ctx=ctx, wall=wall, steps_done=nsteps, t=tnow, dt=dt, kt=kt, ne=info.n_elements, nedge=info.n_edges,
-               np=info.n_basis, nq=info.n_cell_gp, ng=info.n_edge_gp, cfl=info.cfl)
+               np=info.n_basis, nq=info.n_cell_gp, ng=info.n_edge_gp, cfl=info.cfl, troubled=troubled)
     return out
 
 
@@ -89,6 +90,7 @@ def summarise(r, steps, fp32):
     stage_ms = 1e3 * r["wall"] / steps / 3.0
     sb = stage_bytes_survey(r["np"], fp32) * r["ne"]
     return dict(value=value, ms_per_step=1e3 * r["wall"] / steps, dominant=dom, kernel_ms=per,
+                troubled_fraction=sum(r["troubled"]) / (3.0 * r["ne"]), sim_time=r["t"], last_dt=r["dt"],
                 roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
                               traffic=None, kernel=("k_element" if dom == "element" else "k_surface"),
                               bytes_per_launch=kb[dom]),
@@ -159,11 +161,13 @@ def main():
         "roofline": s32["roofline"],
         "stage_roofline": s32["stage"],
         "kernel_ms": s32["kernel_ms"],
+        "troubled_fraction": s32["troubled_fraction"],
         "cpu_baseline": cpu,
     }
     if s64:
         line["fp64"] = {"value": s64["value"], "ms_per_step": s64["ms_per_step"], "roofline": s64["roofline"],
-                        "stage_roofline": s64["stage"], "kernel_ms": s64["kernel_ms"]}
+                        "stage_roofline": s64["stage"], "kernel_ms": s64["kernel_ms"],
+                        "troubled_fraction": s64["troubled_fraction"]}
         line["fp32_over_fp64"] = s32["value"] / s64["value"]
     print(json.dumps(line), flush=True)
 

@@ -23,7 +23,7 @@ EXPORTED = ["dgm_ctx_create", "dgm_ctx_create_host", "dgm_ctx_destroy", "dgm_ctx
             "dgm_compute_residuals", "dgm_apply_limiter", "dgm_compute_time_step", "dgm_rk3", "dgm_time_march",
             "dgm_state_upload", "dgm_state_download", "dgm_device_time_march", "dgm_device_steps_async",
             "dgm_synchronize", "dgm_stream", "dgm_check_state", "dgm_device_clock", "dgm_lift_coefficient",
-            "dgm_profile_steps"]
+            "dgm_profile_steps", "dgm_troubled_counts"]
 
 
 class Mesh(C.Structure):
@@ -108,6 +108,7 @@ def lib():
     L.dgm_check_state.argtypes = [vp]
     L.dgm_device_clock.argtypes = [vp, _f64p, _i32p, _f64p]
     L.dgm_lift_coefficient.argtypes = [vp, C.c_double, _f64p]
+    L.dgm_troubled_counts.argtypes = [vp, _i32p]
     L.dgm_profile_steps.argtypes = [vp, C.c_int32, C.POINTER(KernelTimes)]
     _lib = L
     return L

@@ -310,6 +310,12 @@ int build_device(dgm_ctx* c) {
   S.level_offsets.assign(M.level_offsets.begin(), M.level_offsets.end());
   S.limiter = c->prm.limiter;
   S.hcr = c->prm.hcr;
+  if (c->prm.limiter != DGM_LIMITER_OFF) {
+    if (int s = dalloc(c, &S.smat, (size_t)nel * c->solver->smoothness_len())) return s;
+    c->solver->build_smoothness(S, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   S.weno_grid = 2048;
   S.apply_grid = 1024;
   return write_ctl(c, fresh_ctl(c));
@@ -707,6 +713,14 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   Ctl h;
   if (int s = read_ctl(c, h)) return s;
   return ctl_status(c, h);
+}
+
+int dgm_troubled_counts(dgm_ctx* c, int32_t counts[4]) {
+  if (!c || !counts) return DGM_BAD_ARGUMENT;
+  Ctl h;
+  if (int s = read_ctl(c, h)) return s;
+  for (int i = 0; i < 4; ++i) counts[i] = h.count[i];
+  return DGM_OK;
 }
 
 int dgm_lift_coefficient(dgm_ctx* c, double chord, double* cl) {

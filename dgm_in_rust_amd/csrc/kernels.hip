@@ -591,61 +591,34 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
   }
 }
 
-// WENO rebuild of one troubled element by one 64-lane wavefront (block of 64), FP64 throughout.
-// Lane (face i, polynomial j, variable v) = 16 i + 4 j + v projects polynomial j (0: the target,
-// 1..3: the neighbour across face j-1 with its mean replaced by the target's, weno.rs:148-155) on
-// the characteristic row v of face i (eigenmatrix at the target mean and the raw edge normal,
-// weno.rs:121-124), forms beta = sum_l J^(l-1) sum_{l1} int (d^l P / dx^l1 dy^(l-l1))^2 as P^T B P with
-// the element's smoothness matrix B, and the nonlinear weight linear_w / (beta + eps)^2.
+// Smoothness matrix of one element: beta(P) = sum_{l=1..p} J^(l-1) sum_{l1=0..l} sum_q (J/2) w_q
+// (d^l P / dx^l1 dy^(l-l1))(x_q)^2 (src/spatial_disc/limiter/weno.rs:125-143) is the quadratic form
+// P^T B P with B[k][k'] = sum_l J^(l-1) sum_l1 sum_q (J/2) w_q D_{l1}[q][k] D_{l1}[q][k'], where D are the
+// physical derivative tables of src/mesh.rs:101-112 (binomial expansion of
+// (xx d_xi + ex d_eta)^l1 (xy d_xi + ey d_eta)^l2 for any order). B depends on geometry only, so it is
+// built once per element at set-up (one 64-lane block per element) and stored packed (upper triangle).
 template <int P, int NQ, int NE>
-struct WenoShared {
+struct SmoothShared {
   static constexpr int NP = np_of(P);
   static constexpr int NKW = P * (P + 3) / 2;
-  double src[4][4][NP];
   double D[NKW][NQ][NP];
-  double B[NP][NP];
-  double Pj[3][4][4][NP];
-  double nlw[3][4][4];
-  double Rrow[3][4][4];
-  double PN[3][4][NP];
-  double RES[3][4][NP];
-  double area[3];
-  int nb[3];
-  int edge[3];
-  int flag;
 };
 
-template <class T, int P, int NQ, int NE>
-__device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, double hcr, int e,
-                         WenoShared<P, NQ, NE>& sh, double* __restrict__ out /* [4][NP], global */) {
+template <int P, int NQ, int NE>
+__global__ __launch_bounds__(64) void k_smoothness(DevMesh m, const double* __restrict__ lt, double* __restrict__ smat) {
   using LO = LimOff<P, NQ, NE>;
   constexpr int NP = LO::NP;
   constexpr int NKW = LO::NKW;
+  constexpr int NPK = NP * (NP + 1) / 2;
+  __shared__ SmoothShared<P, NQ, NE> sh;
+  const int e = blockIdx.x;
+  if (e >= m.ne) return;
   const int lane = threadIdx.x;
   const int nes = m.nes;
-  if (lane < 3) {
-    const int nb = m.el_nbr[lane * nes + e];
-    sh.nb[lane] = nb;
-    sh.edge[lane] = m.el_edge[lane * nes + e] >> 1;
-    sh.area[lane] = nb >= 0 ? m.el_geo[G_J * nes + nb] * 0.5 : 0.0;
-  }
-  __syncthreads();
-  for (int idx = lane; idx < 16 * NP; idx += 64) {
-    const int j = idx / (4 * NP), v = (idx / NP) % 4, k = idx % NP;
-    double val = 0.0;
-    if (j == 0 || k == 0)
-      val = (double)U[(size_t)(v * NP + k) * nes + e];
-    else if (sh.nb[j - 1] >= 0)
-      val = (double)U[(size_t)(v * NP + k) * nes + sh.nb[j - 1]];
-    if (j > 0 && sh.nb[j - 1] < 0) val = 0.0;
-    sh.src[j][v][k] = val;
-  }
   const double J = m.el_geo[G_J * nes + e];
   const double xx = m.el_geo[G_XX * nes + e], xy = m.el_geo[G_XY * nes + e];
   const double ex = m.el_geo[G_EX * nes + e], ey = m.el_geo[G_EY * nes + e];
   const double* dref = lt + LO::DREF;
-  // physical derivatives d^l phi_k / dx^l1 dy^(l-l1) at the cell points (src/mesh.rs:101-112 for l <= 2;
-  // binomial expansion of (xx d_xi + ex d_eta)^l1 (xy d_xi + ey d_eta)^l2 in general)
   for (int idx = lane; idx < NKW * NQ * NP; idx += 64) {
     const int kw = idx / (NQ * NP), q = (idx / NP) % NQ, k = idx % NP;
     int l = 1;
@@ -670,10 +643,14 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
     sh.D[kw][q][k] = s;
   }
   __syncthreads();
-  // smoothness matrix B[k][k'] = sum_l J^(l-1) sum_l1 sum_q (J/2) w_q D[q][k] D[q][k']
   const double* cw = lt + LO::CW;
-  for (int idx = lane; idx < NP * NP; idx += 64) {
-    const int k = idx / NP, kk = idx % NP;
+  for (int idx = lane; idx < NPK; idx += 64) {
+    int k = 0, rem = idx;
+    while (rem >= NP - k) {
+      rem -= NP - k;
+      ++k;
+    }
+    const int kk = k + rem;
     double s = 0.0;
     for (int kw = 0; kw < NKW; ++kw) {
       int l = 1;
@@ -684,23 +661,65 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
       for (int q = 0; q < NQ; ++q) a += 0.5 * J * cw[q] * sh.D[kw][q][k] * sh.D[kw][q][kk];
       s += jl * a;
     }
-    sh.B[k][kk] = s;
+    smat[(size_t)e * NPK + idx] = s;
   }
-  // linear weights, weno.rs:93-110 (a boundary face i zeroes weights[i])
-  double lw[4] = {0.997, 0.001, 0.001, 0.001};
-  int nbnd = 0;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-    if (sh.nb[i] < 0) {
-      nbnd += 1;
-      lw[i] = 0.0;
-    }
-  if (nbnd == 0)
-    lw[0] = 0.997;
-  else if (nbnd == 1)
-    lw[1] = 0.998;
-  else if (nbnd == 2)
-    lw[2] = 0.999;
+}
+
+// WENO rebuild of one troubled element by one 64-lane wavefront (block of 64), FP64 throughout.
+// Lane (face i, polynomial j, variable v) = 16 i + 4 j + v projects polynomial j (0: the target,
+// 1..3: the neighbour across face j-1 with its mean replaced by the target's, weno.rs:148-155) on
+// the characteristic row v of face i (eigenmatrix at the target mean and the raw edge normal,
+// weno.rs:121-124), forms beta = P^T B P with the element's smoothness matrix, and the nonlinear
+// weight linear_w / (beta + eps)^2 (weno.rs:142,173).
+template <int P, int NQ, int NE>
+struct WenoShared {
+  static constexpr int NP = np_of(P);
+  static constexpr int NPK = NP * (NP + 1) / 2;
+  double src[4][4][NP];
+  double B[NPK];
+  double Pj[3][4][4][NP];
+  double nlw[3][4][4];
+  double Rrow[3][4][4];
+  double PN[3][4][NP];
+  double RES[3][4][NP];
+  double area[3];
+  int nb[3];
+  int edge[3];
+  int flag;
+};
+
+template <class T, int P, int NQ, int NE>
+__device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ smat, double hcr, int e,
+                         WenoShared<P, NQ, NE>& sh, double* __restrict__ out /* [4][NP] */) {
+  constexpr int NP = np_of(P);
+  constexpr int NPK = NP * (NP + 1) / 2;
+  const int lane = threadIdx.x;
+  const int nes = m.nes;
+  if (lane < 3) {
+    const int nb = m.el_nbr[lane * nes + e];
+    sh.nb[lane] = nb;
+    sh.edge[lane] = m.el_edge[lane * nes + e] >> 1;
+    sh.area[lane] = nb >= 0 ? m.el_geo[G_J * nes + nb] * 0.5 : 0.0;
+  }
+  for (int idx = lane; idx < NPK; idx += 64) sh.B[idx] = smat[(size_t)e * NPK + idx];
+  __syncthreads();
+  for (int idx = lane; idx < 16 * NP; idx += 64) {
+    const int j = idx / (4 * NP), v = (idx / NP) % 4, k = idx % NP;
+    double val = 0.0;
+    if (j == 0 || k == 0)
+      val = (double)U[(size_t)(v * NP + k) * nes + e];
+    else if (sh.nb[j - 1] >= 0)
+      val = (double)U[(size_t)(v * NP + k) * nes + sh.nb[j - 1]];
+    if (j > 0 && sh.nb[j - 1] < 0) val = 0.0;
+    sh.src[j][v][k] = val;
+  }
+  // linear weights, weno.rs:93-110 (a boundary face i zeroes weights[i], then weights[count] is set)
+  const bool b0 = sh.nb[0] < 0, b1 = sh.nb[1] < 0, b2 = sh.nb[2] < 0;
+  const int nbnd = (int)b0 + (int)b1 + (int)b2;
+  double lw0 = b0 ? 0.0 : 0.997, lw1 = b1 ? 0.0 : 0.001, lw2 = b2 ? 0.0 : 0.001, lw3 = 0.001;
+  if (nbnd == 0) lw0 = 0.997;
+  if (nbnd == 1) lw1 = 0.998;
+  if (nbnd == 2) lw2 = 0.999;
   __syncthreads();
   if (nbnd > 2) return 5;  // the reference panics: "Invalid number of boundary neighbours!"
   const double EPS = 2.220446049250313e-16;
@@ -715,24 +734,39 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
       const double mean[4] = {sh.src[0][0][0], sh.src[0][1][0], sh.src[0][2][0], sh.src[0][3][0]};
       double Lm[16], Rm[16];
       eigenmatrix(mean, nx, ny, hcr, Lm, Rm);
+      double lr[4], rr[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        lr[c] = Lm[c];
+        rr[c] = Rm[c];
+      }
+#pragma unroll
+      for (int r = 1; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          lr[c] = v == r ? Lm[4 * r + c] : lr[c];
+          rr[c] = v == r ? Rm[4 * r + c] : rr[c];
+        }
       if (j == 0) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sh.Rrow[i][v][c] = Rm[4 * v + c];
+        for (int c = 0; c < 4; ++c) sh.Rrow[i][v][c] = rr[c];
       }
       if (j == 0 || sh.nb[j - 1] >= 0) {
 #pragma unroll
         for (int k = 0; k < NP; ++k)
-          Pk[k] = Lm[4 * v] * sh.src[j][0][k] + Lm[4 * v + 1] * sh.src[j][1][k] + Lm[4 * v + 2] * sh.src[j][2][k] +
-                  Lm[4 * v + 3] * sh.src[j][3][k];
+          Pk[k] = lr[0] * sh.src[j][0][k] + lr[1] * sh.src[j][1][k] + lr[2] * sh.src[j][2][k] + lr[3] * sh.src[j][3][k];
         double beta = 0.0;
+        int o = 0;
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
-          double t = 0.0;
+          double t = sh.B[o] * Pk[k];
 #pragma unroll
-          for (int kk = 0; kk < NP; ++kk) t += sh.B[k][kk] * Pk[kk];
+          for (int kk = k + 1; kk < NP; ++kk) t += 2.0 * sh.B[o + kk - k] * Pk[kk];
+          o += NP - k;
           beta += Pk[k] * t;
         }
-        w = lw[j] / ((beta + EPS) * (beta + EPS));
+        const double lwj = j == 0 ? lw0 : (j == 1 ? lw1 : (j == 2 ? lw2 : lw3));
+        w = lwj / ((beta + EPS) * (beta + EPS));
       }
     }
 #pragma unroll
@@ -800,7 +834,7 @@ __device__ __forceinline__ void store_limited(const StageBufs<T>& b, int nes, in
 }
 
 template <class T, int P, int NQ, int NE>
-__global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U, const double* __restrict__ lt,
+__global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U, const double* __restrict__ smat,
                                              LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force) {
   __shared__ WenoShared<P, NQ, NE> sh;
   if (!force && (!ctl->active || ctl->error != ~0ull)) return;
@@ -808,7 +842,7 @@ __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U,
   const int n = ctl->count[slot];
   for (int s = blockIdx.x; s < n; s += gridDim.x) {
     const int e = lb.list[s];
-    const int st = weno_wave<T, P, NQ, NE>(m, U, lt, ctl->hcr, e, sh, lb.w + (size_t)s * 4 * NP);
+    const int st = weno_wave<T, P, NQ, NE>(m, U, smat, ctl->hcr, e, sh, lb.w + (size_t)s * 4 * NP);
     if (st && threadIdx.x == 0) atomicMin(&ctl->error, (0xFFFFFFull << 8) | (unsigned long long)st);
     __syncthreads();
   }
@@ -830,7 +864,8 @@ __global__ __launch_bounds__(256) void k_apply(LimiterBufs lb, StageBufs<T> b, i
 template <class T, int P, int NQ, int NE, int STAGE>
 __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __restrict__ elems, int n, StageBufs<T> b,
                                                     const T* __restrict__ U, const double* __restrict__ lt,
-                                                    LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force) {
+                                                    const double* __restrict__ smat, LimiterBufs lb,
+                                                    Ctl* __restrict__ ctl, int slot, int force) {
   __shared__ WenoShared<P, NQ, NE> sh;
   __shared__ double outv[4 * np_of(P)];
   if (!force && (!ctl->active || ctl->error != ~0ull)) return;
@@ -845,7 +880,7 @@ __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __rest
   }
   __syncthreads();
   if (!sh.flag) return;
-  const int st = weno_wave<T, P, NQ, NE>(m, U, lt, ctl->hcr, e, sh, outv);
+  const int st = weno_wave<T, P, NQ, NE>(m, U, smat, ctl->hcr, e, sh, outv);
   __syncthreads();
   if (st) {
     if (threadIdx.x == 0) atomicMin(&ctl->error, ((unsigned long long)(unsigned)e << 8) | (unsigned long long)st);
@@ -988,13 +1023,13 @@ class SolverImpl final : public Solver {
     LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags};
     if (s.limiter == DGM_LIMITER_JACOBI) {
       k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
-      k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
+      k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, force);
       k_apply<T, NP, STAGE><<<s.apply_grid, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot, force);
     } else if (s.limiter == DGM_LIMITER_SEQUENTIAL) {
       for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
         const int n = s.level_offsets[l + 1] - s.level_offsets[l];
         k_limit_level<T, P, NQ, NE, STAGE><<<n, 64, 0, st>>>(s.mesh, s.levels + s.level_offsets[l], n, b, U, s.ltab,
-                                                             lb, s.ctl, slot, force);
+                                                             s.smat, lb, s.ctl, slot, force);
       }
     }
   }
@@ -1041,16 +1076,21 @@ class SolverImpl final : public Solver {
       if (part == PART_DETECT)
         k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
       else if (part == PART_REBUILD)
-        k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
+        k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, force);
       else if (part == PART_APPLY)
         k_apply<T, NP, STAGE><<<s.apply_grid, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot, force);
     } else if (s.limiter == DGM_LIMITER_SEQUENTIAL && part == PART_DETECT) {
       for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
         const int n = s.level_offsets[l + 1] - s.level_offsets[l];
         k_limit_level<T, P, NQ, NE, STAGE><<<n, 64, 0, st>>>(s.mesh, s.levels + s.level_offsets[l], n, b, U, s.ltab,
-                                                             lb, s.ctl, slot, force);
+                                                             s.smat, lb, s.ctl, slot, force);
       }
     }
+  }
+
+  int smoothness_len() const override { return NP * (NP + 1) / 2; }
+  void build_smoothness(const DeviceState& s, hipStream_t st) const override {
+    if (s.mesh.ne > 0) k_smoothness<P, NQ, NE><<<s.mesh.ne, 64, 0, st>>>(s.mesh, s.ltab, s.smat);
   }
 
   void time_step(const DeviceState& s, hipStream_t st) const override {

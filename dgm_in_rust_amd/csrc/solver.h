@@ -22,6 +22,7 @@ struct DeviceState {
   void* flux = nullptr;
   void* tab = nullptr;     // bulk-precision tables
   double* ltab = nullptr;  // FP64 limiter tables
+  double* smat = nullptr;  // per-element packed smoothness matrices [ne][Np(Np+1)/2]
   int* lim_list = nullptr;
   double* lim_w = nullptr;
   uint8_t* lim_flags = nullptr;
@@ -49,6 +50,9 @@ class Solver {
   // element-order sweep), WENO rebuild, apply
   enum { PART_SURFACE = 0, PART_ELEMENT = 1, PART_DETECT = 2, PART_REBUILD = 3, PART_APPLY = 4, N_PARTS = 5 };
   virtual void stage_part(const DeviceState& s, int stage_idx, int part, hipStream_t st) const = 0;
+  // per-element smoothness matrices of the WENO indicator (set-up)
+  virtual int smoothness_len() const = 0;
+  virtual void build_smoothness(const DeviceState& s, hipStream_t st) const = 0;
   // time-step reduction into Ctl::dtmin_bits
   virtual void time_step(const DeviceState& s, hipStream_t st) const = 0;
   virtual void upload(const DeviceState& s, const double* host_layout_dev, hipStream_t st) const = 0;

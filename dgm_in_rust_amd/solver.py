@@ -164,6 +164,11 @@ class Context:
         self.check(self._L.dgm_profile_steps(self.h, n, C.byref(kt)))
         return {k: getattr(kt, k) for k, _ in _lib.KernelTimes._fields_}
 
+    def troubled_counts(self):
+        c = (C.c_int32 * 4)()
+        self.check(self._L.dgm_troubled_counts(self.h, c))
+        return list(c)
+
     def lift_coefficient(self, chord=1.0):
         cl = C.c_double()
         self.check(self._L.dgm_lift_coefficient(self.h, chord, C.byref(cl)))

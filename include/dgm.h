@@ -165,6 +165,8 @@ typedef struct {
   int32_t steps;
 } dgm_kernel_times;
 int dgm_profile_steps(dgm_ctx* ctx, int32_t n_steps, dgm_kernel_times* out);
+/* troubled cells found by the limiter in the last run stage 1, 2, 3 and the stand-alone limiter call */
+int dgm_troubled_counts(dgm_ctx* ctx, int32_t counts[4]);
 /* Lift coefficient Cl = (Fy cos a - Fx sin a) / (0.5 rho V^2 c) of the wall pressure (extension), chord c. */
 int dgm_lift_coefficient(dgm_ctx* ctx, double chord, double* cl);
 
