@@ -124,11 +124,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--traffic", default=None, help="FETCH_KIB,WRITE_KIB per launch of the dominant FP32 kernel")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (counter scale reference for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, device="cuda", dtype=torch.float32)
+        bb = torch.empty_like(a)
+        for _ in range(5):
+            bb.copy_(a)
+        torch.cuda.synchronize()
+        print(f"copy probe: 5 x {args.copy_probe} MiB device copies", file=sys.stderr, flush=True)
     from dgm_in_rust_amd import meshgen
     mach, aoa = 0.8, 1.25
     mesh = meshgen.naca0012_ogrid(args.surface, args.layers, seed=0)

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -310,6 +311,10 @@ int build_device(dgm_ctx* c) {
   S.level_offsets.assign(M.level_offsets.begin(), M.level_offsets.end());
   S.limiter = c->prm.limiter;
   S.hcr = c->prm.hcr;
+  {
+    const char* v = std::getenv("DGM_ELEMENT_KERNEL");  // "valu" selects the one-lane-per-element kernel
+    S.use_mfma = (v && std::strcmp(v, "valu") == 0) ? 0 : 1;
+  }
   if (c->prm.limiter != DGM_LIMITER_OFF) {
     if (int s = dalloc(c, &S.smat, (size_t)nel * c->solver->smoothness_len())) return s;
     c->solver->build_smoothness(S, c->stream);

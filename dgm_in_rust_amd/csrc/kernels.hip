@@ -435,6 +435,165 @@ __global__ __launch_bounds__(128) void k_element(DevMesh m, StageBufs<T> b, cons
 }
 
 // ---------------------------------------------------------------------------------------
+// FP32-bulk element kernel on the matrix cores: one wavefront per 16 elements.
+//
+// With c = lane % 16 (the element e0 + c) and g = lane / 16, each contraction is a
+// v_mfma_f32_16x16x4f32 product over a K slice of 4 (A: lane holds A[c][g]; B: lane holds B[g][c];
+// D: lane holds D[4g + r][c], r = 0..3), written with the element index as the 16 columns:
+//   values    u_v^T[q][e]  = sum_k PHI^T[q][k] U_v^T[k][e]          K slice s: k = 4g + s
+//   volume    R_v^T[k][e] += sum_q DXI^T[k][q] Ft_v^T[q][e] + DETA^T[k][q] Gt_v^T[q][e]
+//                                                                   K slice s: q = 4g + s
+//   surface   R_v^T[k][e] += sum_m' LIFT^T[k][m'] S_v^T[m'][e]      K slice s: m' = (face, point) = 4g + s
+// The K order of the volume product is chosen so that each lane's B operand in slice s is the
+// flux it computed itself at point q = 4g + s (the output rows of the value product): no data
+// moves between lanes. Lanes also hold exactly the (k = 4g + r, e) state entries the RK update
+// needs. Requires Np <= 16, Nq <= 16, 3 Ne <= 16 (p <= 3).
+// ---------------------------------------------------------------------------------------
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <int NP, int NQ, int NE, int OP>
+__global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float> b, const float* __restrict__ G,
+                                                      const float* __restrict__ tab, Ctl* __restrict__ ctl, int force) {
+  static_assert(NP <= 16 && NQ <= 16 && 3 * NE <= 16, "matrix-core element kernel needs p <= 3");
+  using O = TabOff<NP, NQ, NE>;
+  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid == 0) {
+    const int slot = OP == OP_S0 ? 0 : (OP == OP_S1 ? 1 : 2);
+    ctl->count[slot] = 0;
+  }
+  const int lane = threadIdx.x & 63;
+  const int wave = tid >> 6;
+  const int c = lane & 15, g = lane >> 4;
+  const int e = wave * 16 + c;
+  const bool valid = e < m.ne;
+  const int nes = m.nes;
+  const int es = valid ? e : 0;
+  // per-lane table operands (A of each product)
+  float aPhi[4], aDxi[4], aDeta[4], aLift[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = 4 * g + s;  // value product: K index = mode
+    aPhi[s] = (k < NP && c < NQ) ? tab[O::PHIV + c * NP + k] : 0.f;
+    const int q = 4 * g + s;  // volume product: K index = point, row = mode c
+    aDxi[s] = (q < NQ && c < NP) ? tab[O::DXI + q * NP + c] : 0.f;
+    aDeta[s] = (q < NQ && c < NP) ? tab[O::DETA + q * NP + c] : 0.f;
+    const int mp = 4 * g + s;  // surface product: K index = face * NE + point
+    aLift[s] = (mp < 3 * NE && c < NP) ? tab[O::LIFT + (mp / NE * NE + mp % NE) * NP + c] : 0.f;
+  }
+  // state entries held by this lane: U_v[e][k = 4g + s]
+  float us[4][4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * g + s;
+      us[v][s] = (k < NP) ? b.in[(size_t)(v * NP + k) * nes + es] : 0.f;
+    }
+  // values at the cell points q = 4g + r of element e
+  f32x4 uq[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    uq[v] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) uq[v] = mfma4(aPhi[s], us[v][s], uq[v]);
+  }
+  const float* eg = m.el_geo32;
+  const float xx = eg[G_XX * nes + es], xy = eg[G_XY * nes + es], ex = eg[G_EX * nes + es], ey = eg[G_EY * nes + es];
+  const float j2 = eg[G_2J * nes + es];
+  const float hcr = (float)ctl->hcr;
+  // fluxes at the lane's four points, in the metric-transformed form
+  float ft[4][4], gt[4][4];  // [var][r]
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float q4[4] = {uq[0][r], uq[1][r], uq[2][r], uq[3][r]};
+    const bool live = valid && (4 * g + r) < NQ;
+    float f[4], gg[4];
+    euler_flux(q4, hcr, f, gg);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      ft[v][r] = live ? xx * f[v] + xy * gg[v] : 0.f;
+      gt[v][r] = live ? ex * f[v] + ey * gg[v] : 0.f;
+    }
+  }
+  f32x4 acc[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    acc[v] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      acc[v] = mfma4(aDxi[s], ft[v][s], acc[v]);
+      acc[v] = mfma4(aDeta[s], gt[v][s], acc[v]);
+    }
+  }
+  // surface: B operand = signed, 2/J-scaled point flux of (face, point) = 4g + s, point order reversed on the right
+  float sv[4][4];  // [s][var]
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int mp = 4 * g + s;
+    float4 fl = make_float4(0.f, 0.f, 0.f, 0.f);
+    float sc = 0.f;
+    if (valid && mp < 3 * NE) {
+      const int i = mp / NE, pt = mp % NE;
+      const int ed = m.el_edge[i * nes + e];
+      const int side = ed & 1;
+      const int src = side ? (NE - 1 - pt) : pt;
+      fl = *reinterpret_cast<const float4*>(G + ((size_t)(ed >> 1) * NE + src) * 4);
+      sc = side ? j2 : -j2;
+    }
+    sv[s][0] = sc * fl.x;
+    sv[s][1] = sc * fl.y;
+    sv[s][2] = sc * fl.z;
+    sv[s][3] = sc * fl.w;
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc[v] = mfma4(aLift[s], sv[s][v], acc[v]);
+  if (!valid) return;
+  // lane holds R_v[e][k = 4g + r]
+  if (OP == OP_RESID) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 4 * g + r;
+        if (k < NP) b.rout[(size_t)(v * NP + k) * nes + e] = (double)acc[v][r];
+      }
+    return;
+  }
+  const double dt = ctl->dt;
+  const float dtf = (float)dt;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = 4 * g + r;
+      if (k >= NP) continue;
+      const size_t ix = (size_t)(v * NP + k) * nes + e;
+      const float rr = acc[v][r];
+      if (OP == OP_S0) {
+        const float d = dtf * rr;
+        b.d[ix] = d;
+        b.s[ix] = us[v][r] + d;
+      } else if (OP == OP_S1) {
+        const float d = 0.25f * (b.d[ix] + dtf * rr);
+        b.d[ix] = d;
+        b.s[ix] = b.sn[ix] + d;
+      } else {
+        const double d = 2.0 / 3.0 * ((double)b.d[ix] + dt * (double)rr);
+        const double un = b.un[ix] + d;
+        b.un[ix] = un;
+        b.sn[ix] = (float)un;
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // time step: TemperalDisc::compute_time_step (wave speed from the sum of ALL modes, Q5)
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ double wave_min(double v) {
@@ -1005,6 +1164,13 @@ class SolverImpl final : public Solver {
   }
   template <int OP>
   void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force) const {
+    if constexpr (sizeof(T) == 4 && NP <= 16 && NQ <= 16 && 3 * NE <= 16) {
+      if (s.use_mfma) {
+        k_element_mfma<NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, 64), 256, 0, st>>>(s.mesh, b, (const float*)s.flux,
+                                                                                 (const float*)s.tab, s.ctl, force);
+        return;
+      }
+    }
     k_element<T, NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, 128), 128, 0, st>>>(s.mesh, b, (const T*)s.flux,
                                                                             (const T*)s.tab, s.ctl, force);
   }

@@ -28,6 +28,7 @@ struct DeviceState {
   uint8_t* lim_flags = nullptr;
   Ctl* ctl = nullptr;
   int limiter = 0;
+  int use_mfma = 1;        // FP32-bulk element kernel on the matrix cores (p <= 3)
   unsigned weno_grid = 1, apply_grid = 1;
   std::vector<int> level_offsets;
   const int* levels = nullptr;
