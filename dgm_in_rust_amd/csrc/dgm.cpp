@@ -98,7 +98,9 @@ Ctl fresh_ctl(const dgm_ctx* c) {
   h.cfl = c->cfl;
   h.hcr = c->prm.hcr;
   const double big = 1.0e10;
-  std::memcpy(&h.dtmin_bits, &big, 8);
+  for (int sl = 0; sl < 2; ++sl)
+    for (int i = 0; i < DT_LANES * DT_STRIDE; ++i) std::memcpy(&h.dtmin_bits[sl][i], &big, 8);
+  h.pending = 0;
   h.error = ~0ull;
   h.step = 0;
   h.final_step = 0x7fffffff;
@@ -132,11 +134,31 @@ int stage_out(dgm_ctx* c, const double* soa, double* U) {
   return DGM_OK;
 }
 
+// The Jacobi sweep's last stage reduces the next step's time step itself; other modes run the
+// stand-alone reduction at the start of each step.
+bool fused_dt(const dgm_ctx* c) { return c->prm.limiter == DGM_LIMITER_JACOBI; }
+
 void enqueue_step(dgm_ctx* c) {
-  c->solver->time_step(c->S, c->stream);
+  if (!fused_dt(c)) c->solver->time_step(c->S, c->stream);
   launch_step_begin(c->S.ctl, c->stream);
   for (int s = 0; s < 3; ++s) c->solver->stage(c->S, s, c->stream);
-  launch_step_end(c->S.ctl, c->stream);
+}
+
+// fresh control block for a resident run, with the first step's time step already reduced
+int start_run(dgm_ctx* c, int32_t final_step, double final_time) {
+  Ctl h = fresh_ctl(c);
+  h.final_step = final_step;
+  h.final_time = final_time;
+  if (int s = write_ctl(c, h)) return s;
+  if (fused_dt(c)) c->solver->time_step(c->S, c->stream);
+  HIPCHK(c, hipGetLastError());
+  return DGM_OK;
+}
+
+// the step counted at the start of the next one (Ctl::pending) is already done
+void effective_clock(const Ctl& h, double& t, int32_t& steps) {
+  t = h.pending ? h.t + h.dt : h.t;
+  steps = h.step + (h.pending ? 1 : 0);
 }
 
 int ensure_graph(dgm_ctx* c) {
@@ -314,6 +336,8 @@ int build_device(dgm_ctx* c) {
   {
     const char* v = std::getenv("DGM_ELEMENT_KERNEL");  // "valu" selects the one-lane-per-element kernel
     S.use_mfma = (v && std::strcmp(v, "valu") == 0) ? 0 : 1;
+    const char* r = std::getenv("DGM_DETECT_REMAP");
+    S.remap_detect = (r && r[0] == '0') ? 0 : 1;
   }
   if (c->prm.limiter != DGM_LIMITER_OFF) {
     if (int s = dalloc(c, &S.smat, (size_t)nel * c->solver->smoothness_len())) return s;
@@ -527,6 +551,7 @@ int dgm_cell_points_xy(const dgm_ctx* c, double* xy) {
 int dgm_state_upload(dgm_ctx* c, const double* U) {
   if (!c || !U) return DGM_BAD_ARGUMENT;
   if (int s = stage_in(c, U)) return s;
+  if (int s = start_run(c, 0x7fffffff, 1.0e300)) return s;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DGM_OK;
 }
@@ -582,7 +607,13 @@ int dgm_compute_time_step(dgm_ctx* c, const double* U, double* dt) {
   c->solver->time_step(c->S, c->stream);
   HIPCHK(c, hipGetLastError());
   if (int s = read_ctl(c, h)) return s;
-  std::memcpy(dt, &h.dtmin_bits, 8);
+  double m = 1.0e10;
+  for (int i = 0; i < DT_LANES; ++i) {
+    double v;
+    std::memcpy(&v, &h.dtmin_bits[0][i * DT_STRIDE], 8);
+    m = std::fmin(m, v);
+  }
+  *dt = m;
   return DGM_OK;
 }
 
@@ -601,22 +632,23 @@ int dgm_rk3(dgm_ctx* c, double* U, double dt) {
 
 int dgm_device_time_march(dgm_ctx* c, int32_t final_step, double final_time, dgm_march_stats* stats) {
   if (!c) return DGM_BAD_ARGUMENT;
-  Ctl h = fresh_ctl(c);
-  h.final_step = final_step;
-  h.final_time = final_time;
-  if (int s = write_ctl(c, h)) return s;
   if (int s = ensure_graph(c)) return s;
+  if (int s = start_run(c, final_step, final_time)) return s;
   const int batch = 16;
+  Ctl h;
+  double t = 0.0;
+  int32_t steps = 0;
   while (true) {
     for (int i = 0; i < batch; ++i) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     if (int s = read_ctl(c, h)) return s;
-    if (h.error != ~0ull || h.step >= final_step || !(h.t < final_time)) break;
+    effective_clock(h, t, steps);
+    if (h.error != ~0ull || !h.active || steps >= final_step || !(t < final_time)) break;
   }
   if (stats) {
-    stats->steps = h.step;
-    stats->time = h.t;
+    stats->steps = steps;
+    stats->time = t;
     stats->last_dt = h.dt;
-    stats->troubled_cells_last_stage = h.last_count;
+    stats->troubled_cells_last_stage = h.pending ? h.count[2] : h.last_count;
   }
   return ctl_status(c, h);
 }
@@ -660,8 +692,11 @@ int dgm_device_clock(dgm_ctx* c, double* t, int32_t* steps, double* last_dt) {
   if (!c) return DGM_BAD_ARGUMENT;
   Ctl h;
   if (int s = read_ctl(c, h)) return s;
-  if (t) *t = h.t;
-  if (steps) *steps = h.step;
+  double te;
+  int32_t se;
+  effective_clock(h, te, se);
+  if (t) *t = te;
+  if (steps) *steps = se;
   if (last_dt) *last_dt = h.dt;
   return DGM_OK;
 }
@@ -669,7 +704,7 @@ int dgm_device_clock(dgm_ctx* c, double* t, int32_t* steps, double* last_dt) {
 int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   if (!c || !c->stream || !out || n < 1) return DGM_BAD_ARGUMENT;
   std::memset(out, 0, sizeof(*out));
-  const int nev = 2 + 1 + 3 * Solver::N_PARTS + 1 + 1;
+  const int nev = 3 + 3 * Solver::N_PARTS;
   std::vector<hipEvent_t> ev(nev);
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
   double acc[Solver::N_PARTS] = {0, 0, 0, 0, 0};
@@ -677,7 +712,7 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   for (int it = 0; it < n; ++it) {
     int k = 0;
     HIPCHK(c, hipEventRecord(ev[k++], c->stream));
-    c->solver->time_step(c->S, c->stream);
+    if (!fused_dt(c)) c->solver->time_step(c->S, c->stream);
     HIPCHK(c, hipEventRecord(ev[k++], c->stream));
     launch_step_begin(c->S.ctl, c->stream);
     HIPCHK(c, hipEventRecord(ev[k++], c->stream));
@@ -686,8 +721,6 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
         c->solver->stage_part(c->S, st, part, c->stream);
         HIPCHK(c, hipEventRecord(ev[k++], c->stream));
       }
-    launch_step_end(c->S.ctl, c->stream);
-    HIPCHK(c, hipEventRecord(ev[k++], c->stream));
     HIPCHK(c, hipEventSynchronize(ev[k - 1]));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
@@ -700,14 +733,12 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
         HIPCHK(c, hipEventElapsedTime(&ms, ev[idx], ev[idx + 1]));
         acc[part] += ms;
       }
-    HIPCHK(c, hipEventElapsedTime(&ms, ev[idx], ev[idx + 1]));
-    t_ctl += ms;
-    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[idx + 1]));
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[idx]));
     t_step += ms;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   out->time_step_ms = t_dt / n;
-  out->control_ms = t_ctl / (2.0 * n);
+  out->control_ms = t_ctl / n;
   out->surface_ms = acc[0] / (3.0 * n);
   out->element_ms = acc[1] / (3.0 * n);
   out->detect_ms = acc[2] / (3.0 * n);

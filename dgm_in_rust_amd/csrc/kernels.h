@@ -33,13 +33,19 @@ struct DevMesh {
 };
 
 // Step control, resident on the device so that whole steps replay from a captured graph.
+// A step is counted at the start of the next one (pending): the time-step reduction for step n + 1
+// is fused into step n's last limiter pass and lands in dtmin_bits[(n + 1) & 1].
+// The reduction targets are spread over DT_LANES words 128 bytes apart: atomics from many workgroups
+// to one word serialise in L2, so each workgroup / wavefront picks a word by its index.
+constexpr int DT_LANES = 32;
+constexpr int DT_STRIDE = 16;  // unsigned long longs between two targets (128 bytes)
 struct Ctl {
   double t, dt, final_time, cfl, hcr;
-  unsigned long long dtmin_bits;   // atomicMin target of the time-step reduction
-  unsigned long long error;        // (edge << 8) | status of the lowest failing edge, ~0ull when clean
-  int step, final_step, active, pad;
-  int count[4];                    // troubled cells per stage (slot 3: stand-alone limiter call)
-  int last_count;
+  unsigned long long error;         // (edge << 8) | status of the lowest failing edge, ~0ull when clean
+  int step, final_step, active, pending;
+  int count[4];                     // troubled cells per stage (slot 3: stand-alone limiter call)
+  int last_count, pad;
+  unsigned long long dtmin_bits[2][DT_LANES * DT_STRIDE];  // atomicMin targets by step parity
 };
 
 // Buffers the element kernel reads and writes. FP64 mode uses in/un/us; FP32-bulk uses the

@@ -30,6 +30,14 @@ namespace {
 
 constexpr int np_of(int p) { return (p + 1) * (p + 2) / 2; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8). Renumber them so
+// that each XCD works on one contiguous range of logical blocks: neighbouring edges / elements, which
+// gather the same element data, then share that XCD's L2. A bijection on [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // ---- arithmetic helpers: FP64 uses IEEE operations; the FP32 bulk path uses the hardware
 // reciprocal / square root / log2 / exp2 (1 ulp class), which is its stated precision. ----
 __device__ __forceinline__ double dv(double a, double b) { return a / b; }
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
   for (int i = threadIdx.x; i < 3 * NE * NP; i += blockDim.x) sphi[i] = tab[O::PHIE + i];
   __syncthreads();
   if (!force && (!ctl->active || ctl->error != ~0ull)) return;
-  const int ed = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ed = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (ed >= m.nedge) return;
   const int nes = m.nes;
   const int info = m.ed_info[ed];
@@ -320,7 +328,7 @@ __global__ __launch_bounds__(128) void k_element(DevMesh m, StageBufs<T> b, cons
                                                  const T* __restrict__ tab, Ctl* __restrict__ ctl, int force) {
   using O = TabOff<NP, NQ, NE>;
   if (!force && (!ctl->active || ctl->error != ~0ull)) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e == 0) {
     // the limiter of this stage counts troubled cells from zero
     const int slot = OP == OP_S0 ? 0 : (OP == OP_S1 ? 1 : 2);
@@ -421,9 +429,10 @@ __global__ __launch_bounds__(128) void k_element(DevMesh m, StageBufs<T> b, cons
           b.d[ix] = d;
           b.s[ix] = (float)u[v][k] + d;
         } else if (OP == OP_S1) {
-          const float d = 0.25f * (b.d[ix] + dtf * rr);
+          const float d0 = b.d[ix];
+          const float d = 0.25f * (d0 + dtf * rr);
           b.d[ix] = d;
-          b.s[ix] = b.sn[ix] + d;
+          b.s[ix] = ((float)u[v][k] - d0) + d;
         } else {
           const double d = 2.0 / 3.0 * ((double)b.d[ix] + dt * (double)rr);
           const double un = b.un[ix] + d;
@@ -461,7 +470,7 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
   static_assert(NP <= 16 && NQ <= 16 && 3 * NE <= 16, "matrix-core element kernel needs p <= 3");
   using O = TabOff<NP, NQ, NE>;
   if (!force && (!ctl->active || ctl->error != ~0ull)) return;
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (tid == 0) {
     const int slot = OP == OP_S0 ? 0 : (OP == OP_S1 ? 1 : 2);
     ctl->count[slot] = 0;
@@ -581,9 +590,11 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
         b.d[ix] = d;
         b.s[ix] = us[v][r] + d;
       } else if (OP == OP_S1) {
-        const float d = 0.25f * (b.d[ix] + dtf * rr);
+        // u^(2) shadow = u^(1) shadow - d1 + d2 (= s^n + d2 up to float rounding); no s^n re-read
+        const float d0 = b.d[ix];
+        const float d = 0.25f * (d0 + dtf * rr);
         b.d[ix] = d;
-        b.s[ix] = b.sn[ix] + d;
+        b.s[ix] = (us[v][r] - d0) + d;
       } else {
         const double d = 2.0 / 3.0 * ((double)b.d[ix] + dt * (double)rr);
         const double un = b.un[ix] + d;
@@ -602,56 +613,73 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
+// dt of one element, src/temporal_disc.rs:63-81: wave speed of the state given by the SUM of all modes
+template <int NP, class F>
+__device__ __forceinline__ double element_dt(F coef, double hcr, double cfl, double hmin) {
+  double s[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    double a = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) a += coef(v, k);
+    s[v] = a;
+  }
+  const double u = s[1] / s[0], v = s[2] / s[0];
+  const double p = (hcr - 1.0) * (s[3] - 0.5 * s[0] * (u * u + v * v));
+  const double c = sqrt(hcr * p / s[0]);
+  const double speed = sqrt(u * u + v * v) + c;
+  return cfl * hmin / speed;
+}
+
+// block minimum of per-thread dt candidates, then one atomicMin per block (positive doubles order
+// like their bit patterns); a NaN candidate is never taken, as in `if dt < time_step` of the reference
+// every lane of the wavefront must reach this call
+__device__ __forceinline__ void wave_min_dt(double dt, unsigned long long* slot_base, int wave_index) {
+  dt = wave_min(dt);
+  if ((threadIdx.x & 63) == 0 && dt < 1.0e10)
+    atomicMin(&slot_base[(wave_index % DT_LANES) * DT_STRIDE], (unsigned long long)__double_as_longlong(dt));
+}
+
+__device__ __forceinline__ double read_dt_slot(const unsigned long long* slot_base) {
+  double m = 1.0e10;
+  for (int i = 0; i < DT_LANES; ++i) m = fmin(m, __longlong_as_double((long long)slot_base[i * DT_STRIDE]));
+  return m;
+}
+
+// stand-alone time-step reduction into the slot of the step about to begin
 template <class T, int NP>
 __global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, Ctl* __restrict__ ctl) {
-  __shared__ double red[4];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   double dt = 1.0e10;
   if (e < m.ne) {
     const int nes = m.nes;
-    double s[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double a = 0.0;
-#pragma unroll
-      for (int k = 0; k < NP; ++k) a += (double)U[(size_t)(v * NP + k) * nes + e];
-      s[v] = a;
-    }
-    const double hcr = ctl->hcr;
-    const double u = s[1] / s[0], v = s[2] / s[0];
-    const double p = (hcr - 1.0) * (s[3] - 0.5 * s[0] * (u * u + v * v));
-    const double c = sqrt(hcr * p / s[0]);
-    const double speed = sqrt(u * u + v * v) + c;
-    const double d = ctl->cfl * m.el_geo[G_HMIN * nes + e] / speed;
-    if (d < dt) dt = d;  // a NaN candidate is never taken, as in the reference's `if dt < time_step`
+    const double d = element_dt<NP>([&](int v, int k) { return (double)U[(size_t)(v * NP + k) * nes + e]; }, ctl->hcr,
+                                    ctl->cfl, m.el_geo[G_HMIN * nes + e]);
+    if (d < dt) dt = d;
   }
-  dt = wave_min(dt);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (lane == 0) red[wv] = dt;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = red[0];
-    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) b = fmin(b, red[i]);
-    atomicMin(&ctl->dtmin_bits, (unsigned long long)__double_as_longlong(b));
-  }
+  wave_min_dt(dt, ctl->dtmin_bits[(ctl->step + ctl->pending) & 1], (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
 }
 
-// while step < final_step && t < final_time (src/temporal_disc.rs:22-26)
+// Step control (src/temporal_disc.rs:22-34): count the step that just ran, take its successor's dt
+// (clamped to final_time), and decide whether it runs (step < final_step && t < final_time).
 __global__ void k_step_begin(Ctl* ctl) {
-  const double dtm = __longlong_as_double((long long)ctl->dtmin_bits);
-  ctl->dtmin_bits = (unsigned long long)__double_as_longlong(1.0e10);
-  const bool go = ctl->step < ctl->final_step && ctl->t < ctl->final_time && ctl->error == ~0ull;
-  ctl->active = go ? 1 : 0;
-  double dt = dtm;
-  if (ctl->t + dt > ctl->final_time) dt = ctl->final_time - ctl->t;
-  if (go) ctl->dt = dt;
-}
-
-__global__ void k_step_end(Ctl* ctl) {
-  if (ctl->active && ctl->error == ~0ull) {
+  if (ctl->pending) {
     ctl->t += ctl->dt;
     ctl->step += 1;
+    ctl->pending = 0;
     ctl->last_count = ctl->count[2];
+  }
+  const int slot = ctl->step & 1;
+  const double dtm = read_dt_slot(ctl->dtmin_bits[slot]);
+  for (int i = 0; i < DT_LANES; ++i)
+    ctl->dtmin_bits[slot][i * DT_STRIDE] = (unsigned long long)__double_as_longlong(1.0e10);
+  const bool go = ctl->step < ctl->final_step && ctl->t < ctl->final_time && ctl->error == ~0ull;
+  ctl->active = go ? 1 : 0;
+  if (go) {
+    double dt = dtm;
+    if (ctl->t + dt > ctl->final_time) dt = ctl->final_time - ctl->t;
+    ctl->dt = dt;
+    ctl->pending = 1;
   }
 }
 
@@ -665,7 +693,8 @@ __global__ void k_reset_count(Ctl* ctl, int slot) { ctl->count[slot] = 0; }
 // local edge index with reversed point order (:21-27, :63-64); faces without a neighbour are skipped;
 // no inflow face gives 0/0 = NaN, which is not >= 1.
 template <class T, int P, int NQ, int NE>
-__device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, int e) {
+__device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, int e,
+                      double* mode_sums = nullptr) {
   using LO = LimOff<P, NQ, NE>;
   constexpr int NP = LO::NP;
   const int nes = m.nes;
@@ -676,6 +705,20 @@ __device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* _
     ru[k] = (double)U[(size_t)(1 * NP + k) * nes + e];
     rv[k] = (double)U[(size_t)(2 * NP + k) * nes + e];
     en[k] = (double)U[(size_t)(3 * NP + k) * nes + e];
+  }
+  if (mode_sums) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      a0 += rho[k];
+      a1 += ru[k];
+      a2 += rv[k];
+      a3 += en[k];
+    }
+    mode_sums[0] = a0;
+    mode_sums[1] = a1;
+    mode_sums[2] = a2;
+    mode_sums[3] = a3;
   }
   double djump = 0.0, ejump = 0.0, surface = 0.0, minr = 1.0e10, mine = 1.0e10;
   const double* phie = lt + LO::PHIE;
@@ -736,18 +779,41 @@ __device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* _
   return (id >= 1.0) || (ie >= 1.0);
 }
 
+// KXRCF detector, one lane per element. With fuse_dt (the last stage of a step) the next step's time
+// step is reduced here for the cells that stay untouched; k_weno adds the rebuilt ones.
 template <class T, int P, int NQ, int NE>
 __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ U, const double* __restrict__ lt,
-                                               LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force) {
+                                               LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force, int fuse_dt,
+                                               int remap) {
+  constexpr int NP = np_of(P);
   if (!force && (!ctl->active || ctl->error != ~0ull)) return;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m.ne) return;
-  const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e);
-  if (lb.flags) lb.flags[e] = t ? 1 : 0;
-  if (t) {
-    const int i = atomicAdd(&ctl->count[slot], 1);
-    lb.list[i] = e;
+  const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int e = blk * blockDim.x + threadIdx.x;
+  if (!fuse_dt) {
+    if (e >= m.ne) return;
+    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e);
+    if (lb.flags) lb.flags[e] = t ? 1 : 0;
+    if (t) {
+      const int i = atomicAdd(&ctl->count[slot], 1);
+      lb.list[i] = e;
+    }
+    return;
   }
+  double dt = 1.0e10;
+  if (e < m.ne) {
+    double sums[4];
+    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e, sums);
+    if (lb.flags) lb.flags[e] = t ? 1 : 0;
+    if (t) {
+      const int i = atomicAdd(&ctl->count[slot], 1);
+      lb.list[i] = e;
+    } else {
+      const double d = element_dt<1>([&](int v, int) { return sums[v]; }, ctl->hcr, ctl->cfl,
+                                     m.el_geo[G_HMIN * m.nes + e]);
+      if (d < dt) dt = d;
+    }
+  }
+  wave_min_dt(dt, ctl->dtmin_bits[(ctl->step + 1) & 1], (blk * (int)blockDim.x + (int)threadIdx.x) >> 6);
 }
 
 // Smoothness matrix of one element: beta(P) = sum_{l=1..p} J^(l-1) sum_{l1=0..l} sum_q (J/2) w_q
@@ -994,15 +1060,24 @@ __device__ __forceinline__ void store_limited(const StageBufs<T>& b, int nes, in
 
 template <class T, int P, int NQ, int NE>
 __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U, const double* __restrict__ smat,
-                                             LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force) {
+                                             LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force, int fuse_dt) {
   __shared__ WenoShared<P, NQ, NE> sh;
   if (!force && (!ctl->active || ctl->error != ~0ull)) return;
   constexpr int NP = np_of(P);
   const int n = ctl->count[slot];
   for (int s = blockIdx.x; s < n; s += gridDim.x) {
     const int e = lb.list[s];
-    const int st = weno_wave<T, P, NQ, NE>(m, U, smat, ctl->hcr, e, sh, lb.w + (size_t)s * 4 * NP);
+    double* out = lb.w + (size_t)s * 4 * NP;
+    const int st = weno_wave<T, P, NQ, NE>(m, U, smat, ctl->hcr, e, sh, out);
     if (st && threadIdx.x == 0) atomicMin(&ctl->error, (0xFFFFFFull << 8) | (unsigned long long)st);
+    __syncthreads();
+    if (fuse_dt && !st && threadIdx.x == 0) {
+      const double d = element_dt<NP>([&](int v, int k) { return out[v * NP + k]; }, ctl->hcr, ctl->cfl,
+                                      m.el_geo[G_HMIN * m.nes + e]);
+      if (d < 1.0e10)
+        atomicMin(&ctl->dtmin_bits[(ctl->step + 1) & 1][(e % DT_LANES) * DT_STRIDE],
+                  (unsigned long long)__double_as_longlong(d));
+    }
     __syncthreads();
   }
 }
@@ -1121,7 +1196,6 @@ inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs
 }  // namespace
 
 void launch_step_begin(Ctl* ctl, hipStream_t st) { k_step_begin<<<1, 1, 0, st>>>(ctl); }
-void launch_step_end(Ctl* ctl, hipStream_t st) { k_step_end<<<1, 1, 0, st>>>(ctl); }
 
 // ---------------------------------------------------------------------------------------
 // Launch sequences for one (precision, order, rule) combination
@@ -1188,8 +1262,9 @@ class SolverImpl final : public Solver {
     const T* U = limiter_input(s, STAGE);
     LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags};
     if (s.limiter == DGM_LIMITER_JACOBI) {
-      k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
-      k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, force);
+      k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force, 0,
+                                                                         s.remap_detect);
+      k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, force, 0);
       k_apply<T, NP, STAGE><<<s.apply_grid, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot, force);
     } else if (s.limiter == DGM_LIMITER_SEQUENTIAL) {
       for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
@@ -1239,10 +1314,12 @@ class SolverImpl final : public Solver {
     const T* U = limiter_input(s, STAGE);
     LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags};
     if (s.limiter == DGM_LIMITER_JACOBI) {
+      const int fuse = STAGE == 2 ? 1 : 0;  // the last stage also reduces the next step's dt
       if (part == PART_DETECT)
-        k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force);
+        k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force,
+                                                                         fuse, s.remap_detect);
       else if (part == PART_REBUILD)
-        k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, force);
+        k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, force, fuse);
       else if (part == PART_APPLY)
         k_apply<T, NP, STAGE><<<s.apply_grid, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot, force);
     } else if (s.limiter == DGM_LIMITER_SEQUENTIAL && part == PART_DETECT) {

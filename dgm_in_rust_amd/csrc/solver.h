@@ -29,6 +29,7 @@ struct DeviceState {
   Ctl* ctl = nullptr;
   int limiter = 0;
   int use_mfma = 1;        // FP32-bulk element kernel on the matrix cores (p <= 3)
+  int remap_detect = 1;    // XCD-contiguous block order for the detector
   unsigned weno_grid = 1, apply_grid = 1;
   std::vector<int> level_offsets;
   const int* levels = nullptr;
@@ -66,6 +67,5 @@ std::unique_ptr<Solver> make_solver(int precision, int p, int nq, int ne);
 
 // kernel launches with single-thread control
 void launch_step_begin(Ctl* ctl, hipStream_t st);
-void launch_step_end(Ctl* ctl, hipStream_t st);
 
 }  // namespace dgm
