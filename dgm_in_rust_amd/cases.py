@@ -1,0 +1,69 @@
+"""Flow cases of the benchmark configurations and their error norms.
+
+* Isentropic vortex (strength beta, centre (x0, y0), mean flow (u_inf, v_inf), rho = p = 1 at
+  infinity) on the doubly periodic square: the exact solution is the initial field advected
+  with the mean flow. Not part of the reference (which has no periodic boundary); it pins the
+  whole scheme's accuracy.
+* Uniform freestream (the reference's initial condition, src/solver.rs:45-79).
+
+Norms are evaluated with the cell quadrature rule of the context: for a field q given at the
+cell points, ||q||_2^2 = sum_e sum_q (J_e / 2) w_q q^2.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def vortex_primitive(x, y, t=0.0, beta=5.0, x0=5.0, y0=5.0, u_inf=1.0, v_inf=1.0, length=10.0, hcr=1.4):
+    """Exact (rho, u, v, p) of the isentropic vortex at time t on a periodic square of side length."""
+    xc = x0 + u_inf * t
+    yc = y0 + v_inf * t
+    # nearest periodic image of the centre
+    dx = (x - xc + 0.5 * length) % length - 0.5 * length
+    dy = (y - yc + 0.5 * length) % length - 0.5 * length
+    r2 = dx * dx + dy * dy
+    f = np.exp(0.5 * (1.0 - r2))
+    du = -beta / (2.0 * np.pi) * f * dy
+    dv = beta / (2.0 * np.pi) * f * dx
+    T = 1.0 - (hcr - 1.0) * beta * beta / (8.0 * hcr * np.pi * np.pi) * f * f
+    rho = T ** (1.0 / (hcr - 1.0))
+    p = rho ** hcr
+    return rho, u_inf + du, v_inf + dv, p
+
+
+def conservative(rho, u, v, p, hcr=1.4):
+    return np.stack([rho, rho * u, rho * v, p / (hcr - 1.0) + 0.5 * rho * (u * u + v * v)], axis=-1)
+
+
+def vortex_state(ctx, t=0.0, **kw):
+    """L2 projection of the exact vortex at time t onto the element modes of ``ctx``."""
+    xy = ctx.cell_points_xy()
+    q = conservative(*vortex_primitive(xy[..., 0], xy[..., 1], t=t, **kw), hcr=kw.get("hcr", 1.4))
+    return ctx.project(q)
+
+
+def point_values(ctx, U):
+    """[ne, nq, 4] values of the modal state U at the cell points."""
+    phi = ctx.tables()["phi_cell"]                 # [nq, nb]
+    return np.einsum("evk,qk->eqv", U, phi)
+
+
+def l2_norm_of_points(ctx, q):
+    """Per-variable L2 norm of point values q [ne, nq, 4]."""
+    w = ctx.tables()["cell_w"]
+    J = ctx.geometry()["jacob_det"]
+    return np.sqrt(np.einsum("e,q,eqv->v", 0.5 * J, w, q * q))
+
+
+def vortex_error(ctx, U, t, **kw):
+    """Per-variable L2 error of U against the exact vortex at time t, and the density error."""
+    xy = ctx.cell_points_xy()
+    exact = conservative(*vortex_primitive(xy[..., 0], xy[..., 1], t=t, **kw), hcr=kw.get("hcr", 1.4))
+    return l2_norm_of_points(ctx, point_values(ctx, U) - exact)
+
+
+def relative_l2(ctx, U, Uref):
+    """||U - Uref||_2 / ||Uref||_2 over all variables, with the cell quadrature."""
+    d = l2_norm_of_points(ctx, point_values(ctx, U - Uref))
+    r = l2_norm_of_points(ctx, point_values(ctx, Uref))
+    return float(np.sqrt((d * d).sum() / (r * r).sum()))

@@ -1588,6 +1588,9 @@ void ref_divide_by_mass(const ref_case* rc, double* R) { divide_by_mass(rc->c, R
 
 int32_t ref_apply_limiter(ref_case* rc, double* U, uint8_t* troubled) { return apply_limiter(rc->c, U, troubled); }
 int32_t ref_kxrcf(const ref_case* rc, const double* U, int64_t e) { return modified_kxrcf(rc->c, U, e) ? 1 : 0; }
+int32_t ref_weno_element(const ref_case* rc, const double* U, int64_t e, double* out) {
+  return weno_element(rc->c, U, e, out);
+}
 double ref_compute_time_step(const ref_case* rc, const double* U) { return compute_time_step(rc->c, U); }
 int32_t ref_rk3(ref_case* rc, double* R, double* U, double dt) { return rk3(rc->c, R, U, dt); }
 

@@ -89,6 +89,8 @@ void ref_divide_by_mass(const ref_case* c, double* R);
  * receives the KXRCF flags that were acted on. */
 int32_t ref_apply_limiter(ref_case* c, double* U, uint8_t* troubled);
 int32_t ref_kxrcf(const ref_case* c, const double* U, int64_t ielem);
+// WENO reconstruction of one element from U (weno.rs:86-269 for one troubled cell), U untouched.
+int32_t ref_weno_element(const ref_case* c, const double* U, int64_t ielem, double* out);
 double ref_compute_time_step(const ref_case* c, const double* U);
 int32_t ref_rk3(ref_case* c, double* R, double* U, double dt);
 int32_t ref_time_march(ref_case* c, double* R, double* U, int32_t final_step, double final_time, int32_t* steps_done,

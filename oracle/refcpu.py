@@ -72,6 +72,7 @@ def lib():
         L.ref_divide_by_mass.argtypes = [C.c_void_p, _f64p]
         L.ref_apply_limiter.argtypes = [C.c_void_p, _f64p, _u8p]
         L.ref_kxrcf.argtypes = [C.c_void_p, _f64p, C.c_int64]
+        L.ref_weno_element.argtypes = [C.c_void_p, _f64p, C.c_int64, _f64p]
         L.ref_compute_time_step.restype = C.c_double
         L.ref_compute_time_step.argtypes = [C.c_void_p, _f64p]
         L.ref_rk3.argtypes = [C.c_void_p, _f64p, _f64p, C.c_double]
@@ -231,6 +232,13 @@ class RefCase:
     def kxrcf(self, U, e):
         U = np.ascontiguousarray(U, np.float64)
         return bool(self._L.ref_kxrcf(self.h, _p(U, _f64p), e))
+
+    def weno_element(self, U, e):
+        """[4, nb] WENO reconstruction of element e from U (status 0 on success)."""
+        U = np.ascontiguousarray(U, np.float64)
+        out = np.zeros((4, self.nbasis))
+        s = self._L.ref_weno_element(self.h, _p(U, _f64p), e, _p(out, _f64p))
+        return out, s
 
     def compute_time_step(self, U):
         U = np.ascontiguousarray(U, np.float64)

@@ -147,6 +147,73 @@ def test_discrete_conservation_periodic(order):
     assert np.all(np.abs(tot) <= 1e-13 * scale)
 
 
+# ---- KXRCF / WENO invariants ------------------------------------------------------------------
+
+def _scaled_mode0_err(V, U):
+    return float((np.abs(V[:, :, 0] - U[:, :, 0]).max(axis=0) / np.abs(U[:, :, 0]).max(axis=0)).max())
+
+
+@pytest.mark.parametrize("limiter", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_weno_preserves_cell_means(order, limiter):
+    """Every candidate polynomial carries the target's mean (mode 0 swap, weno.rs:148-155), the
+    nonlinear weights are normalised and the face average is area-normalised, so the limited cell
+    mean equals the old one; untroubled cells are left bit-identical."""
+    ref = RefCase(meshgen.naca0012_ogrid(16, 4, seed=11), order=order, limiter=limiter, mach=0.5)
+    U = flow_state(ref, seed=3, shock=True, mach=0.5, noise=0.03)
+    V, tr, s = ref.apply_limiter(U)
+    assert s == 0 and tr.sum() > 0
+    assert _scaled_mode0_err(V, U) <= 1e-13
+    assert np.array_equal(V[~tr], U[~tr])
+    assert np.all(np.abs(V[tr] - U[tr]).max(axis=(1, 2)) > 0)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_weno_identical_neighbours_is_identity(order):
+    """When every element holds the same modal vector, the four candidate polynomials of a face are
+    equal, so any normalised weights give back R L u = u. (KXRCF never flags such a state: with the
+    target's local edge on the neighbour's modes the jump is f(s) - f(1 - s), whose integral under
+    the symmetric edge rule is zero, so the reconstruction is called per element.)"""
+    mesh = meshgen.periodic_square(4, length=0.5, jitter=0.1, seed=12)
+    ref = RefCase(mesh, order=order, limiter=1, mach=0.5)
+    nb = (order + 1) * (order + 2) // 2
+    rng = np.random.default_rng(5)
+    base = np.zeros((4, nb))
+    base[:, 0] = [1.0, 150.0, 60.0, 2.6e5]
+    base[:, 1:] = rng.uniform(-1, 1, (4, nb - 1)) * np.array([0.05, 40.0, 40.0, 5e3])[:, None]
+    U = np.broadcast_to(base, (ref.ne, 4, nb)).copy()
+    assert not any(ref.kxrcf(U, e) for e in range(ref.ne))
+    for e in range(ref.ne):
+        out, s = ref.weno_element(U, e)
+        assert s == 0
+        assert rel_err(out[None], base[None]) <= 1e-13
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_weno_reconstruction_depends_on_neighbours(order):
+    """Planted-mistake guard for the test above: with distinct neighbour vectors the reconstruction
+    moves the higher modes but keeps the mean."""
+    ref = RefCase(meshgen.periodic_square(4, length=0.5, jitter=0.1, seed=12), order=order, limiter=1, mach=0.5)
+    U = flow_state(ref, seed=2, shock=True, mach=0.5, noise=0.05)
+    moved = 0
+    for e in range(ref.ne):
+        out, s = ref.weno_element(U, e)
+        assert s == 0
+        assert np.all(np.abs(out[:, 0] - U[e, :, 0]) <= 1e-13 * np.abs(U[:, :, 0]).max(axis=0))
+        moved += int(np.abs(out[:, 1:] - U[e, :, 1:]).max() > 1e-10 * np.abs(U[e]).max())
+    assert moved > ref.ne // 2
+
+
+def test_uniform_state_is_untroubled():
+    """A uniform state has zero jumps on every face: KXRCF flags nothing (faces with no inflow give
+    0/0, which is not troubled) and the limiter returns the state bit-identical."""
+    for order in (1, 2, 3):
+        ref = RefCase(meshgen.naca0012_ogrid(16, 4, seed=13), order=order, limiter=1, mach=0.7, aoa_deg=3.0)
+        U = ref.initial_solution(1.0, 3.0, 0.7, 1.0e5)
+        V, tr, s = ref.apply_limiter(U)
+        assert s == 0 and tr.sum() == 0 and np.array_equal(V, U)
+
+
 # ---- second restatement ----------------------------------------------------------------------
 
 LIMS = {"sequential": 1, "jacobi": 2}
