@@ -12,8 +12,9 @@ weights and positivity tests); the plain FP64 path is measured in the same run.
 GDOF * stage / s = n_elements * 4 * n_basis * 3 * steps / seconds / 1e9.
 
 The roofline object is for the dominant kernel, with its algorithmic bytes per launch and its
-mean launch time from HIP events on the context stream (dgm_profile_steps); traffic, when given
-(--traffic FETCH_KIB,WRITE_KIB from rocprofv3 counter runs), is reported as measured.
+mean launch time from HIP events on the context stream (dgm_profile_steps); traffic is the HBM
+bytes per dispatch that rocprofv3's FETCH_SIZE / WRITE_SIZE counters measured for the same kernel,
+read from the workload's committed summary under profiles/ (null when there is none).
 The CPU baseline times the oracle (FP64 C++ restatement of the reference, OpenMP in the three
 residual folds like the reference's rayon folds) on a few steps of the same case.
 """
@@ -31,22 +32,25 @@ METRIC = "GDOF·RK-stage/s at p=3 (FP32-bulk vs FP64) + p50/p99 case latency"
 
 
 def kernel_bytes(np_, ne_pts, tsize, nel, nedge):
-    """Algorithmic bytes per launch: every array the kernel must read or write, counted once.
+    """Algorithmic bytes per launch: every array the kernel must read or write, each counted once.
 
-    element (per element): stage input 4Np*T, its three edges' point fluxes 3*Ne*4*T, metric and 2/J
-    5*T, edge ids 3*4, and the RK stage combination averaged over the three stages
-    (FP64: write; read u^n + write; read u^n + write u^n.  FP32-bulk: write d, s; read d, s^n + write
-    d, s; read d, u^n(8) + write u^n(8), s^n).
-    surface (per edge): both neighbours' modes 2*4Np*T, two element ids + info word 12, normal and
-    half length 3*T, and the point fluxes Ne*4*T written.
+    element: stage input 4Np*T per element; the point fluxes of every edge once, nedge*Ne*4*T (an
+    interior edge's flux serves both neighbours); metric and 2/J 5*T and three edge ids 3*4 per
+    element; the RK stage combination averaged over the three stages, per element
+      FP64:      write u^(1); read u^n, write u^(2); read u^n, write u^(n+1)   -> (1 + 2 + 2) * 4Np*8 / 3
+      FP32-bulk: write d, s; read d, write d, s; read d and u^n (FP64), write u^n (FP64) and s^n
+                                                          -> (2*4 + 3*4 + (4 + 8 + 8 + 4)) * 4Np / 3.
+    surface: the modes of every element once, 4Np*T each (an element's modes serve its three edges);
+    two element ids and the info word 12, normal and half length 3*T, and the point fluxes Ne*4*T
+    written, per edge.
     """
     n = 4 * np_
     if tsize == 8:
         rk = (n * 8 + 2 * n * 8 + 2 * n * 8) / 3.0
     else:
-        rk = (2 * n * 4 + 4 * n * 4 + (n * 4 + n * 8 + n * 8 + n * 4)) / 3.0
-    element = nel * (n * tsize + 3 * ne_pts * 4 * tsize + 5 * tsize + 12 + rk)
-    surface = nedge * (2 * n * tsize + 12 + 3 * tsize + ne_pts * 4 * tsize)
+        rk = (2 * n * 4 + 3 * n * 4 + (n * 4 + n * 8 + n * 8 + n * 4)) / 3.0
+    element = nel * (n * tsize + 5 * tsize + 12 + rk) + nedge * ne_pts * 4 * tsize
+    surface = nel * n * tsize + nedge * (12 + 3 * tsize + ne_pts * 4 * tsize)
     return {"element": element, "surface": surface}
 
 
@@ -92,10 +96,28 @@ def summarise(r, steps, fp32):
     return dict(value=value, ms_per_step=1e3 * r["wall"] / steps, dominant=dom, kernel_ms=per,
                 troubled_fraction=sum(r["troubled"]) / (3.0 * r["ne"]), sim_time=r["t"], last_dt=r["dt"],
                 roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                              traffic=None, kernel=("k_element" if dom == "element" else "k_surface"),
+                              traffic=None, kernel=(("k_element_mfma" if fp32 else "k_element") if dom == "element" else "k_surface"),
                               bytes_per_launch=kb[dom]),
                 stage=dict(bytes_per_element_stage=stage_bytes_survey(r["np"], fp32), ms=stage_ms,
                            achieved=sb / (stage_ms * 1e-3) / 1e9, frac=sb / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+
+
+def attach_traffic(summ, r, args, fp32):
+    """roofline.traffic from the committed rocprofv3 counter summary of this workload, when there is one:
+    HBM bytes per dispatch of the dominant kernel (mean over its three stage variants), with the
+    profile's mean duration beside the HIP-event one for comparison."""
+    from dgm_in_rust_amd.rocprof_summary import read_traffic
+    name = f"kernels_naca0012_{args.surface}x{args.layers}_p{args.order}.txt"
+    dims = f"{r['np']},{r['nq']},{r['ng']}"
+    if summ["dominant"] == "element":
+        prefix = f"k_element_mfma<{dims}," if fp32 else f"k_element<double,{dims},"
+    else:
+        prefix = f"k_surface<{'float' if fp32 else 'double'},{dims}>"
+    got = read_traffic(os.path.join(ROOT, "profiles", name), prefix)
+    if got:
+        summ["roofline"]["traffic"] = got[0]
+        summ["roofline"]["profile_avg_us"] = got[1]
+        summ["roofline"]["traffic_source"] = f"profiles/{name}"
 
 
 def cpu_baseline(mesh, order, mach, aoa, nsteps):
@@ -123,7 +145,6 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--traffic", default=None, help="FETCH_KIB,WRITE_KIB per launch of the dominant FP32 kernel")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (counter scale reference for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -153,9 +174,9 @@ def main():
         s64 = summarise(r64, args.steps, False)
         print(f"fp64: {s64['value']:.2f} GDOF*stage/s, {s64['ms_per_step']:.4f} ms/step, kernels {s64['kernel_ms']}",
               file=sys.stderr, flush=True)
-    if args.traffic:
-        f, w = [float(x) for x in args.traffic.split(",")]
-        s32["roofline"]["traffic"] = (f + w) * 1024.0
+    attach_traffic(s32, r32, args, True)
+    if s64:
+        attach_traffic(s64, r64, args, False)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args.order, mach, aoa, args.cpu_steps)
