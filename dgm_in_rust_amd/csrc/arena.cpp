@@ -1,0 +1,488 @@
+// Arena buffers, range allocation and stream-ordered case placement (see arena.h).
+#include "arena.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace dgm {
+
+// ---- RangeAlloc ----
+
+void RangeAlloc::reset(int64_t capacity) {
+  free_.clear();
+  used_.clear();
+  cap_ = capacity;
+  if (capacity > 0) free_[0] = capacity;
+}
+
+int64_t RangeAlloc::alloc(int64_t n) {
+  if (n <= 0) return -1;
+  for (auto it = free_.begin(); it != free_.end(); ++it) {
+    if (it->second < n) continue;
+    const int64_t start = it->first, len = it->second;
+    free_.erase(it);
+    if (len > n) free_[start + n] = len - n;
+    used_[start] = n;
+    return start;
+  }
+  return -1;
+}
+
+void RangeAlloc::release(int64_t start, int64_t n) {
+  auto u = used_.find(start);
+  if (u == used_.end() || u->second != n) return;
+  used_.erase(u);
+  int64_t s = start, len = n;
+  auto next = free_.lower_bound(s);
+  if (next != free_.end() && next->first == s + len) {
+    len += next->second;
+    next = free_.erase(next);
+  }
+  if (next != free_.begin()) {
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == s) {
+      s = prev->first;
+      len += prev->second;
+      free_.erase(prev);
+    }
+  }
+  free_[s] = len;
+}
+
+int64_t RangeAlloc::used_end() const { return used_.empty() ? 0 : used_.rbegin()->first + used_.rbegin()->second; }
+
+int64_t RangeAlloc::free_units() const {
+  int64_t f = 0;
+  for (const auto& kv : free_) f += kv.second;
+  return f;
+}
+
+int64_t RangeAlloc::largest_free() const {
+  int64_t f = 0;
+  for (const auto& kv : free_) f = std::max(f, kv.second);
+  return f;
+}
+
+// ---- helpers ----
+
+std::vector<double> farfield_states(const HostMesh& M, double mach, double aoa_deg, double rho, double p, double fs_hcr,
+                                    double hcr) {
+  const double cinf = std::sqrt(fs_hcr * p / rho);
+  const double rad = aoa_deg * (M_PI / 180.0);
+  const double uinf = mach * cinf * std::cos(rad), vinf = mach * cinf * std::sin(rad);
+  std::vector<double> q;
+  for (size_t b = 0; b < M.patch_type.size(); ++b) {
+    q.push_back(rho);
+    q.push_back(rho * uinf);
+    q.push_back(rho * vinf);
+    q.push_back(p / (hcr - 1.0) + 0.5 * rho * (uinf * uinf + vinf * vinf));
+  }
+  return q;
+}
+
+CaseCtl fresh_case(double cfl) {
+  CaseCtl h;
+  std::memset(&h, 0, sizeof(h));
+  h.final_time = 1.0e300;
+  h.cfl = cfl;
+  h.error = ~0ull;
+  h.final_step = 0x7fffffff;
+  h.active = 1;
+  h.live = 1;
+  return h;
+}
+
+// ---- Arena ----
+
+Footprint Arena::footprint(int order, int nq, int ne_pts, int precision, int limiter, bool residual_buffer) {
+  (void)nq;
+  const int64_t np = (order + 1) * (order + 2) / 2;
+  const int64_t nc = 4 * np;
+  const bool f32 = precision == DGM_FP32_BULK;
+  Footprint f;
+  int64_t e = 0;
+  e += nc * 8;                            // u^n
+  e += f32 ? 3 * nc * 4 : nc * 8;         // shadows + increment, or the FP64 stage state
+  if (residual_buffer) e += nc * 8;       // residual output
+  e += 3 * 4 + 3 * 4;                     // edge ids, neighbours
+  e += G_N * 8 + G_N * 4;                 // geometry (FP64 + FP32 copy)
+  e += 4 + nc * 8 + 1;                    // troubled list, rebuilt polynomials, flags
+  if (limiter != DGM_LIMITER_OFF) e += np * (np + 1) / 2 * 8;  // smoothness matrix
+  e += 1;                                 // case map (4 bytes per 64 elements, rounded up)
+  f.per_element = e;
+  int64_t d = 0;
+  d += 2 * 4 + 4;                         // elements, info word
+  d += 3 * 8 + 3 * 4;                     // normal and half length (FP64 + FP32)
+  d += (int64_t)ne_pts * 4 * (f32 ? 4 : 8);  // point fluxes
+  d += 1;                                 // case map
+  f.per_edge = d;
+  f.per_patch = 4 * 8;
+  return f;
+}
+
+template <class X>
+int Arena::dalloc(X** p, size_t n, std::string& err) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  void* v = nullptr;
+  const hipError_t e = hipMalloc(&v, n * sizeof(X));
+  if (e != hipSuccess) {
+    err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return DGM_OUT_OF_MEMORY;
+  }
+  allocs_.push_back(v);
+  bytes += (int64_t)(n * sizeof(X));
+  *p = (X*)v;
+  return DGM_OK;
+}
+
+#define ACHK(call)                                                   \
+  do {                                                               \
+    hipError_t e_ = (call);                                          \
+    if (e_ != hipSuccess) {                                          \
+      err = std::string(#call ": ") + hipGetErrorString(e_);         \
+      return DGM_HIP_ERROR;                                          \
+    }                                                                \
+  } while (0)
+
+Arena::~Arena() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  for (void* p : allocs_) (void)hipFree(p);
+  if (pinned_) (void)hipHostFree(pinned_);
+}
+
+int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, int64_t cap_elements, int64_t cap_edges,
+                int64_t cap_patches, bool residual_buffer, CaseCtl* cases, unsigned long long* dtmin, hipStream_t st,
+                std::string& err) {
+  order = order_;
+  precision = precision_;
+  limiter = limiter_;
+  stream = st;
+  if (!build_tables(T, order, nq, ne_pts)) {
+    err = "unsupported quadrature rule";
+    return DGM_BAD_CONFIG;
+  }
+  solver = make_solver(precision, order, nq, ne_pts);
+  if (!solver) {
+    err = "no kernel instantiation for this order and rule";
+    return DGM_BAD_CONFIG;
+  }
+  cap_e = std::max<int64_t>(64, (cap_elements + 63) / 64 * 64);
+  cap_ed = std::max<int64_t>(64, (cap_edges + 63) / 64 * 64);
+  cap_patch = std::max<int64_t>(1, cap_patches);
+  if (cap_e >= (1 << 30) || cap_ed >= (1 << 29) || cap_patch >= (1 << 22)) {
+    err = "arena capacity beyond the 32-bit index range";
+    return DGM_BAD_CONFIG;
+  }
+  r_e.reset(cap_e / 64);
+  r_ed.reset(cap_ed / 64);
+  r_patch.reset(cap_patch);
+  const int np = T.np, nqq = T.nq, nep = T.ne;
+  const bool f32 = solver->fp32();
+
+  // bulk tables: PHIV, DXI, DETA [nq][np] (derivatives pre-divided by the mass, times the weight);
+  // PHIE, LIFT [3][ne][np]
+  std::vector<double> tab(solver->table_len(), 0.0);
+  if ((int)tab.size() != 3 * nqq * np + 6 * nep * np) {
+    err = "table size mismatch";
+    return DGM_BAD_CONFIG;
+  }
+  const int kxi = T.key(1, 0), keta = T.key(0, 1);
+  for (int q = 0; q < nqq; ++q)
+    for (int k = 0; k < np; ++k) {
+      tab[q * np + k] = T.phi_cell[q * np + k];
+      tab[nqq * np + q * np + k] = T.cell_w[q] * T.dref[((size_t)kxi * nqq + q) * np + k] / T.mass[k];
+      tab[2 * nqq * np + q * np + k] = T.cell_w[q] * T.dref[((size_t)keta * nqq + q) * np + k] / T.mass[k];
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int m = 0; m < nep; ++m)
+      for (int k = 0; k < np; ++k) {
+        const double ph = T.phi_edge[(i * nep + m) * np + k];
+        tab[3 * nqq * np + (i * nep + m) * np + k] = ph;
+        tab[3 * nqq * np + 3 * nep * np + (i * nep + m) * np + k] = T.edge_w[m] * ph / T.mass[k];
+      }
+  if (f32) {
+    std::vector<float> t32(tab.begin(), tab.end());
+    float* p;
+    if (int s = dalloc(&p, t32.size(), err)) return s;
+    ACHK(hipMemcpy(p, t32.data(), t32.size() * 4, hipMemcpyHostToDevice));
+    S.tab = p;
+  } else {
+    double* p;
+    if (int s = dalloc(&p, tab.size(), err)) return s;
+    ACHK(hipMemcpy(p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    S.tab = p;
+  }
+  // FP64 limiter tables: PHIE [3][ne][np], EW [ne], CW [nq], DREF
+  std::vector<double> lt;
+  lt.insert(lt.end(), T.phi_edge.begin(), T.phi_edge.end());
+  lt.insert(lt.end(), T.edge_w.begin(), T.edge_w.end());
+  lt.insert(lt.end(), T.cell_w.begin(), T.cell_w.end());
+  lt.insert(lt.end(), T.dref.begin(), T.dref.end());
+  if ((int)lt.size() != solver->limiter_table_len()) {
+    err = "limiter table size mismatch";
+    return DGM_BAD_CONFIG;
+  }
+  if (int s = dalloc(&S.ltab, lt.size(), err)) return s;
+  ACHK(hipMemcpy(S.ltab, lt.data(), lt.size() * 8, hipMemcpyHostToDevice));
+
+  // mesh arrays at capacity
+  DevMesh& dm = S.mesh;
+  dm.ne = 0;
+  dm.nes = (int)cap_e;
+  dm.nedge = 0;
+  dm.ed_stride = (int)cap_ed;
+  dm.npatch = (int)cap_patch;
+  int *el_edge, *el_nbr, *ed_elem, *ed_info, *elc, *edc;
+  double *el_geo, *ed_geo, *pq;
+  float *el_geo32, *ed_geo32;
+  if (int s = dalloc(&el_edge, 3 * cap_e, err)) return s;
+  if (int s = dalloc(&el_nbr, 3 * cap_e, err)) return s;
+  if (int s = dalloc(&el_geo, (size_t)G_N * cap_e, err)) return s;
+  if (int s = dalloc(&el_geo32, (size_t)G_N * cap_e, err)) return s;
+  if (int s = dalloc(&ed_elem, 2 * cap_ed, err)) return s;
+  if (int s = dalloc(&ed_info, cap_ed, err)) return s;
+  if (int s = dalloc(&ed_geo, 3 * cap_ed, err)) return s;
+  if (int s = dalloc(&ed_geo32, 3 * cap_ed, err)) return s;
+  if (int s = dalloc(&pq, 4 * cap_patch, err)) return s;
+  if (int s = dalloc(&elc, cap_e / 64, err)) return s;
+  if (int s = dalloc(&edc, cap_ed / 64, err)) return s;
+  ACHK(hipMemset(el_edge, 0, 3 * cap_e * 4));
+  ACHK(hipMemset(el_nbr, 0xff, 3 * cap_e * 4));
+  ACHK(hipMemset(el_geo, 0, G_N * cap_e * 8));
+  ACHK(hipMemset(el_geo32, 0, G_N * cap_e * 4));
+  ACHK(hipMemset(ed_elem, 0xff, 2 * cap_ed * 4));
+  ACHK(hipMemset(ed_info, 0, cap_ed * 4));
+  ACHK(hipMemset(ed_geo, 0, 3 * cap_ed * 8));
+  ACHK(hipMemset(ed_geo32, 0, 3 * cap_ed * 4));
+  ACHK(hipMemset(pq, 0, 4 * cap_patch * 8));
+  ACHK(hipMemset(elc, 0xff, cap_e / 64 * 4));
+  ACHK(hipMemset(edc, 0xff, cap_ed / 64 * 4));
+  dm.el_edge = el_edge;
+  dm.el_nbr = el_nbr;
+  dm.el_geo = el_geo;
+  dm.el_geo32 = el_geo32;
+  dm.ed_elem = ed_elem;
+  dm.ed_info = ed_info;
+  dm.ed_geo = ed_geo;
+  dm.ed_geo32 = ed_geo32;
+  dm.patch_q = pq;
+  dm.el_case64 = elc;
+  dm.ed_case64 = edc;
+
+  // state and work buffers
+  const size_t ns = (size_t)4 * np * cap_e;
+  if (int s = dalloc(&S.un, ns, err)) return s;
+  ACHK(hipMemset(S.un, 0, ns * 8));
+  if (f32) {
+    float *a, *b, *d;
+    if (int s = dalloc(&a, ns, err)) return s;
+    if (int s = dalloc(&b, ns, err)) return s;
+    if (int s = dalloc(&d, ns, err)) return s;
+    ACHK(hipMemset(a, 0, ns * 4));
+    ACHK(hipMemset(b, 0, ns * 4));
+    ACHK(hipMemset(d, 0, ns * 4));
+    S.sn = a;
+    S.s = b;
+    S.d = d;
+  } else {
+    double* a;
+    if (int s = dalloc(&a, ns, err)) return s;
+    ACHK(hipMemset(a, 0, ns * 8));
+    S.us = a;
+  }
+  if (residual_buffer) {
+    if (int s = dalloc(&S.rout, ns, err)) return s;
+  }
+  char* fl;
+  if (int s = dalloc(&fl, (size_t)cap_ed * nep * 4 * (f32 ? 4 : 8), err)) return s;
+  ACHK(hipMemset(fl, 0, (size_t)cap_ed * nep * 4 * (f32 ? 4 : 8)));
+  S.flux = fl;
+  if (int s = dalloc(&S.lim_list, cap_e, err)) return s;
+  if (int s = dalloc(&S.lim_w, (size_t)cap_e * 4 * np, err)) return s;
+  if (int s = dalloc(&S.lim_flags, cap_e, err)) return s;
+  if (limiter != DGM_LIMITER_OFF) {
+    if (int s = dalloc(&S.smat, (size_t)cap_e * solver->smoothness_len(), err)) return s;
+  }
+  int* count;
+  if (int s = dalloc(&count, 4, err)) return s;
+  ACHK(hipMemset(count, 0, 16));
+  S.ctl.cases = cases;
+  S.ctl.dtmin = dtmin;
+  S.ctl.count = count;
+  S.limiter = limiter;
+  S.use_mfma = solver->has_mfma() ? 1 : 0;
+  dm.hcr = 1.4;
+  return DGM_OK;
+}
+
+// Copies (dst, src, bytes) pieces through one pinned staging buffer on the arena's stream, then waits.
+int Arena::stage_copy(const std::vector<std::pair<void*, std::pair<const void*, size_t>>>& pieces, std::string& err) {
+  size_t total = 0;
+  for (const auto& p : pieces) total += (p.second.second + 255) / 256 * 256;
+  if (total > pinned_size_) {
+    if (pinned_) ACHK(hipHostFree(pinned_));
+    pinned_ = nullptr;
+    ACHK(hipHostMalloc(&pinned_, total, hipHostMallocDefault));
+    pinned_size_ = total;
+  }
+  size_t off = 0;
+  for (const auto& p : pieces) {
+    const size_t n = p.second.second;
+    if (n == 0) continue;
+    std::memcpy((char*)pinned_ + off, p.second.first, n);
+    ACHK(hipMemcpyAsync(p.first, (char*)pinned_ + off, n, hipMemcpyHostToDevice, stream));
+    off += (n + 255) / 256 * 256;
+  }
+  ACHK(hipStreamSynchronize(stream));
+  return DGM_OK;
+}
+
+int Arena::set_case64(int* map, int64_t first, int64_t n, int value, std::string& err) {
+  std::vector<int> v((size_t)n, value);
+  return stage_copy({{map + first, {v.data(), v.size() * 4}}}, err);
+}
+
+int Arena::place(const HostMesh& M, const std::vector<double>& patch_q, int case_id, Placement& out, std::string& err) {
+  Placement pl;
+  pl.ne = M.ne;
+  pl.ned = M.nedge;
+  pl.npatch = (int64_t)M.patch_type.size();
+  pl.ne_alloc = (pl.ne + 63) / 64 * 64;
+  pl.ned_alloc = (pl.ned + 63) / 64 * 64;
+  const int64_t be = r_e.alloc(pl.ne_alloc / 64);
+  if (be < 0) return DGM_NO_SPACE;
+  const int64_t bd = r_ed.alloc(pl.ned_alloc / 64);
+  if (bd < 0) {
+    r_e.release(be, pl.ne_alloc / 64);
+    return DGM_NO_SPACE;
+  }
+  const int64_t bp = pl.npatch > 0 ? r_patch.alloc(pl.npatch) : 0;
+  if (bp < 0) {
+    r_e.release(be, pl.ne_alloc / 64);
+    r_ed.release(bd, pl.ned_alloc / 64);
+    return DGM_NO_SPACE;
+  }
+  pl.e_off = be * 64;
+  pl.ed_off = bd * 64;
+  pl.patch_off = bp;
+  const int64_t ne = pl.ne, ned = pl.ned, eo = pl.e_off, edo = pl.ed_off;
+  const int64_t nes = cap_e, est = cap_ed;
+  // element arrays, rows of the [k][capacity] layouts
+  std::vector<int> el_edge(3 * ne), el_nbr(3 * ne);
+  std::vector<double> el_geo((size_t)G_N * ne);
+  for (int64_t e = 0; e < ne; ++e) {
+    for (int i = 0; i < 3; ++i) {
+      const int64_t id = M.el_edge[3 * e + i];
+      const int side = (M.ed_elem[2 * id + 1] == e && M.ed_lic[2 * id + 1] == i) ? 1 : 0;
+      el_edge[i * ne + e] = (int)((edo + id) << 1) | side;
+      const int64_t nb = M.el_nbr[3 * e + i];
+      el_nbr[i * ne + e] = nb < 0 ? -1 : (int)(eo + nb);
+    }
+    const double J = M.el_J[e];
+    el_geo[G_J * ne + e] = J;
+    el_geo[G_XX * ne + e] = M.el_metric[4 * e + 0];
+    el_geo[G_XY * ne + e] = M.el_metric[4 * e + 1];
+    el_geo[G_EX * ne + e] = M.el_metric[4 * e + 2];
+    el_geo[G_EY * ne + e] = M.el_metric[4 * e + 3];
+    el_geo[G_CR * ne + e] = M.el_cr[e];
+    el_geo[G_HMIN * ne + e] = M.el_hmin[e];
+    el_geo[G_2J * ne + e] = 2.0 / J;
+  }
+  std::vector<float> el_geo32(el_geo.begin(), el_geo.end());
+  std::vector<int> ed_elem(2 * ned), ed_info(ned);
+  std::vector<double> ed_geo(3 * ned);
+  for (int64_t i = 0; i < ned; ++i) {
+    ed_elem[i] = (int)(eo + M.ed_elem[2 * i]);
+    ed_elem[ned + i] = M.ed_elem[2 * i + 1] < 0 ? -1 : (int)(eo + M.ed_elem[2 * i + 1]);
+    const int licL = (int)M.ed_lic[2 * i];
+    if (M.ed_patch[i] < 0) {
+      ed_info[i] = licL | ((int)M.ed_lic[2 * i + 1] << 2) | (BC_INTERIOR << 4);
+      ed_geo[i] = M.ed_normal[2 * i];
+      ed_geo[ned + i] = M.ed_normal[2 * i + 1];
+    } else {
+      const int patch = (int)M.ed_patch[i];
+      const int bc = M.patch_type[patch] == DGM_WALL ? BC_WALL : BC_FARFIELD;
+      ed_info[i] = licL | (bc << 4) | ((int)(pl.patch_off + patch) << 8);
+      ed_geo[i] = M.ed_bc_normal[2 * i];
+      ed_geo[ned + i] = M.ed_bc_normal[2 * i + 1];
+    }
+    ed_geo[2 * ned + i] = 0.5 * M.ed_len[i];
+  }
+  std::vector<float> ed_geo32(ed_geo.begin(), ed_geo.end());
+  DevMesh& dm = S.mesh;
+  std::vector<std::pair<void*, std::pair<const void*, size_t>>> pc;
+  for (int i = 0; i < 3; ++i) {
+    pc.push_back({(void*)(dm.el_edge + i * nes + eo), {el_edge.data() + i * ne, ne * 4}});
+    pc.push_back({(void*)(dm.el_nbr + i * nes + eo), {el_nbr.data() + i * ne, ne * 4}});
+  }
+  for (int g = 0; g < G_N; ++g) {
+    pc.push_back({(void*)(dm.el_geo + g * nes + eo), {el_geo.data() + g * ne, ne * 8}});
+    pc.push_back({(void*)(dm.el_geo32 + g * nes + eo), {el_geo32.data() + g * ne, ne * 4}});
+  }
+  for (int i = 0; i < 2; ++i) pc.push_back({(void*)(dm.ed_elem + i * est + edo), {ed_elem.data() + i * ned, ned * 4}});
+  pc.push_back({(void*)(dm.ed_info + edo), {ed_info.data(), ned * 4}});
+  for (int i = 0; i < 3; ++i) {
+    pc.push_back({(void*)(dm.ed_geo + i * est + edo), {ed_geo.data() + i * ned, ned * 8}});
+    pc.push_back({(void*)(dm.ed_geo32 + i * est + edo), {ed_geo32.data() + i * ned, ned * 4}});
+  }
+  if (pl.npatch > 0)
+    pc.push_back({(void*)(dm.patch_q + 4 * pl.patch_off), {patch_q.data(), (size_t)pl.npatch * 4 * 8}});
+  std::vector<int> elc((size_t)(pl.ne_alloc / 64), case_id), edc((size_t)(pl.ned_alloc / 64), case_id);
+  if (int s = stage_copy(pc, err)) return s;
+  if (limiter != DGM_LIMITER_OFF) {
+    solver->build_smoothness(S, (int)eo, (int)ne, stream);
+    ACHK(hipGetLastError());
+  }
+  // the blocks become the case's last, after its data (stream order)
+  if (int s = stage_copy({{(void*)(const_cast<int*>(dm.el_case64) + eo / 64), {elc.data(), elc.size() * 4}},
+                          {(void*)(const_cast<int*>(dm.ed_case64) + edo / 64), {edc.data(), edc.size() * 4}}},
+                         err))
+    return s;
+  out = pl;
+  return DGM_OK;
+}
+
+int Arena::unplace(const Placement& pl, std::string& err) {
+  DevMesh& dm = S.mesh;
+  if (int s = set_case64(const_cast<int*>(dm.el_case64), pl.e_off / 64, pl.ne_alloc / 64, -1, err)) return s;
+  if (int s = set_case64(const_cast<int*>(dm.ed_case64), pl.ed_off / 64, pl.ned_alloc / 64, -1, err)) return s;
+  r_e.release(pl.e_off / 64, pl.ne_alloc / 64);
+  r_ed.release(pl.ed_off / 64, pl.ned_alloc / 64);
+  if (pl.npatch > 0) r_patch.release(pl.patch_off, pl.npatch);
+  return DGM_OK;
+}
+
+int Arena::upload_state(const double* U, const Placement& pl, std::string& err) {
+  const size_t n = (size_t)pl.ne * 4 * T.np;
+  if (n > hbuf_size_) {
+    if (int s = dalloc(&hbuf_, n, err)) return s;
+    hbuf_size_ = n;
+  }
+  ACHK(hipMemcpyAsync(hbuf_, U, n * 8, hipMemcpyHostToDevice, stream));
+  solver->upload(S, hbuf_, (int)pl.e_off, (int)pl.ne, stream);
+  ACHK(hipGetLastError());
+  ACHK(hipStreamSynchronize(stream));
+  return DGM_OK;
+}
+
+int Arena::download_state(const double* soa, double* U, const Placement& pl, std::string& err) {
+  const size_t n = (size_t)pl.ne * 4 * T.np;
+  if (n > hbuf_size_) {
+    if (int s = dalloc(&hbuf_, n, err)) return s;
+    hbuf_size_ = n;
+  }
+  solver->download(S, soa, hbuf_, (int)pl.e_off, (int)pl.ne, stream);
+  ACHK(hipGetLastError());
+  ACHK(hipMemcpyAsync(U, hbuf_, n * 8, hipMemcpyDeviceToHost, stream));
+  ACHK(hipStreamSynchronize(stream));
+  return DGM_OK;
+}
+
+void Arena::update_extent() {
+  S.mesh.ne = (int)(r_e.used_end() * 64);
+  S.mesh.nedge = (int)(r_ed.used_end() * 64);
+}
+
+}  // namespace dgm

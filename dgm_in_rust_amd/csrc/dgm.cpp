@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "arena.h"
 #include "dgm_internal.h"
 #include "solver.h"
 
@@ -18,22 +19,21 @@ using namespace dgm;
 
 struct dgm_ctx {
   dgm_params prm{};
-  Tables T;
+  Tables T;        // host copy for the host-side entry points
   HostMesh M;
-  std::unique_ptr<Solver> solver;
-  DeviceState S;
   hipStream_t stream = nullptr;
-  std::vector<void*> allocs;
-  int64_t bytes = 0;
-  double* hbuf = nullptr;     // device staging buffer in host layout [ne][4][np]
-  double* d_force = nullptr;  // [2]
+  std::unique_ptr<Arena> A;  // device buffers: one arena holding this one case
+  Placement pl;
+  CaseCtl* d_case = nullptr;                // [1]
+  unsigned long long* d_dtmin = nullptr;    // [DT_CASE]
+  double* d_force = nullptr;                // [2]
   int* d_flag = nullptr;
   int* d_levels = nullptr;
+  std::vector<void*> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   dgm_error err{};
   double cfl = 0.0;
-  int nes = 0;
 };
 
 namespace {
@@ -62,54 +62,47 @@ int dalloc(dgm_ctx* c, X** p, size_t n) {
   const hipError_t e = hipMalloc(&v, n * sizeof(X));
   if (e != hipSuccess) return fail(c, DGM_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
   c->allocs.push_back(v);
-  c->bytes += (int64_t)(n * sizeof(X));
   *p = (X*)v;
   return DGM_OK;
 }
 
-template <class X>
-int upload(dgm_ctx* c, X** p, const std::vector<X>& h) {
-  int s = dalloc(c, p, h.size());
-  if (s) return s;
-  if (!h.empty()) HIPCHK(c, hipMemcpy(*p, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice));
-  return DGM_OK;
-}
-
 size_t nstate(const dgm_ctx* c) { return (size_t)c->M.ne * 4 * c->T.np; }
+DeviceState& dev(dgm_ctx* c) { return c->A->S; }
+const Solver& solver(dgm_ctx* c) { return *c->A->solver; }
 
-int read_ctl(dgm_ctx* c, Ctl& h) {
+int read_ctl(dgm_ctx* c, CaseCtl& h) {
   if (!c->stream) return fail(c, DGM_BAD_ARGUMENT, "host-only context");
-  HIPCHK(c, hipMemcpyAsync(&h, c->S.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&h, c->d_case, sizeof(CaseCtl), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DGM_OK;
 }
-int write_ctl(dgm_ctx* c, const Ctl& h) {
-  HIPCHK(c, hipMemcpyAsync(c->S.ctl, &h, sizeof(Ctl), hipMemcpyHostToDevice, c->stream));
+// writes the control record and empties both dt slots
+int write_ctl(dgm_ctx* c, const CaseCtl& h) {
+  if (!c->stream) return fail(c, DGM_BAD_ARGUMENT, "host-only context");
+  static std::vector<unsigned long long> empty;
+  if (empty.empty()) {
+    const double big = 1.0e10;
+    unsigned long long b;
+    std::memcpy(&b, &big, 8);
+    empty.assign(DT_CASE, b);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_case, &h, sizeof(CaseCtl), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_dtmin, empty.data(), DT_CASE * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DGM_OK;
 }
 
-Ctl fresh_ctl(const dgm_ctx* c) {
-  Ctl h;
-  std::memset(&h, 0, sizeof(h));
-  h.t = 0.0;
-  h.dt = 0.0;
-  h.final_time = 1.0e300;
-  h.cfl = c->cfl;
-  h.hcr = c->prm.hcr;
-  const double big = 1.0e10;
-  for (int sl = 0; sl < 2; ++sl)
-    for (int i = 0; i < DT_LANES * DT_STRIDE; ++i) std::memcpy(&h.dtmin_bits[sl][i], &big, 8);
-  h.pending = 0;
-  h.error = ~0ull;
-  h.step = 0;
-  h.final_step = 0x7fffffff;
-  h.active = 1;
+CaseCtl fresh_ctl(const dgm_ctx* c) {
+  CaseCtl h = fresh_case(c->cfl);
+  h.e_off = 0;
+  h.e_end = (int)c->M.ne;
+  h.ed_off = 0;
+  h.ed_end = (int)c->M.nedge;
   return h;
 }
 
 // status word of the device: lowest failing edge and its code
-int ctl_status(dgm_ctx* c, const Ctl& h) {
+int ctl_status(dgm_ctx* c, const CaseCtl& h) {
   if (h.error == ~0ull) return DGM_OK;
   const int code = (int)(h.error & 0xff);
   const int64_t edge = (int64_t)(h.error >> 8);
@@ -121,16 +114,13 @@ int ctl_status(dgm_ctx* c, const Ctl& h) {
 
 int stage_in(dgm_ctx* c, const double* U) {
   if (!c->stream) return fail(c, DGM_BAD_ARGUMENT, "host-only context");
-  HIPCHK(c, hipMemcpyAsync(c->hbuf, U, nstate(c) * 8, hipMemcpyHostToDevice, c->stream));
-  c->solver->upload(c->S, c->hbuf, c->stream);
-  HIPCHK(c, hipGetLastError());
+  std::string e;
+  if (int s = c->A->upload_state(U, c->pl, e)) return fail(c, s, e);
   return DGM_OK;
 }
 int stage_out(dgm_ctx* c, const double* soa, double* U) {
-  c->solver->download(c->S, soa, c->hbuf, c->stream);
-  HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(U, c->hbuf, nstate(c) * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::string e;
+  if (int s = c->A->download_state(soa, U, c->pl, e)) return fail(c, s, e);
   return DGM_OK;
 }
 
@@ -139,24 +129,24 @@ int stage_out(dgm_ctx* c, const double* soa, double* U) {
 bool fused_dt(const dgm_ctx* c) { return c->prm.limiter == DGM_LIMITER_JACOBI; }
 
 void enqueue_step(dgm_ctx* c) {
-  if (!fused_dt(c)) c->solver->time_step(c->S, c->stream);
-  launch_step_begin(c->S.ctl, c->stream);
-  for (int s = 0; s < 3; ++s) c->solver->stage(c->S, s, c->stream);
+  if (!fused_dt(c)) solver(c).time_step(dev(c), c->stream, 0);
+  launch_step_begin(dev(c).ctl, 1, c->stream);
+  for (int s = 0; s < 3; ++s) solver(c).stage(dev(c), s, c->stream);
 }
 
 // fresh control block for a resident run, with the first step's time step already reduced
 int start_run(dgm_ctx* c, int32_t final_step, double final_time) {
-  Ctl h = fresh_ctl(c);
+  CaseCtl h = fresh_ctl(c);
   h.final_step = final_step;
   h.final_time = final_time;
   if (int s = write_ctl(c, h)) return s;
-  if (fused_dt(c)) c->solver->time_step(c->S, c->stream);
+  if (fused_dt(c)) solver(c).time_step(dev(c), c->stream, 0);
   HIPCHK(c, hipGetLastError());
   return DGM_OK;
 }
 
-// the step counted at the start of the next one (Ctl::pending) is already done
-void effective_clock(const Ctl& h, double& t, int32_t& steps) {
+// the step counted at the start of the next one (CaseCtl::pending) is already done
+void effective_clock(const CaseCtl& h, double& t, int32_t& steps) {
   t = h.pending ? h.t + h.dt : h.t;
   steps = h.step + (h.pending ? 1 : 0);
 }
@@ -172,182 +162,36 @@ int ensure_graph(dgm_ctx* c) {
 }
 
 int build_device(dgm_ctx* c) {
-  const Tables& T = c->T;
-  const HostMesh& M = c->M;
-  const int np = T.np, nq = T.nq, ne = T.ne;
-  const int nes = (int)((M.ne + 63) / 64 * 64);
-  c->nes = nes;
-  DeviceState& S = c->S;
-  const bool f32 = c->solver->fp32();
-  // bulk tables: PHIV, DXI, DETA [nq][np]; PHIE, LIFT [3][ne][np]
-  std::vector<double> tab(c->solver->table_len(), 0.0);
-  if ((int)tab.size() != 3 * nq * np + 6 * ne * np) return fail(c, DGM_BAD_CONFIG, "table size mismatch");
-  const int kxi = T.key(1, 0), keta = T.key(0, 1);
-  for (int q = 0; q < nq; ++q)
-    for (int k = 0; k < np; ++k) {
-      tab[q * np + k] = T.phi_cell[q * np + k];
-      tab[nq * np + q * np + k] = T.cell_w[q] * T.dref[((size_t)kxi * nq + q) * np + k] / T.mass[k];
-      tab[2 * nq * np + q * np + k] = T.cell_w[q] * T.dref[((size_t)keta * nq + q) * np + k] / T.mass[k];
-    }
-  for (int i = 0; i < 3; ++i)
-    for (int m = 0; m < ne; ++m)
-      for (int k = 0; k < np; ++k) {
-        const double ph = T.phi_edge[(i * ne + m) * np + k];
-        tab[3 * nq * np + (i * ne + m) * np + k] = ph;
-        tab[3 * nq * np + 3 * ne * np + (i * ne + m) * np + k] = T.edge_w[m] * ph / T.mass[k];
-      }
-  if (f32) {
-    std::vector<float> t32(tab.begin(), tab.end());
-    float* p;
-    if (int s = upload(c, &p, t32)) return s;
-    S.tab = p;
-  } else {
-    double* p;
-    if (int s = upload(c, &p, tab)) return s;
-    S.tab = p;
-  }
-  // FP64 limiter tables: PHIE [3][ne][np], EW [ne], CW [nq], DREF
-  std::vector<double> lt;
-  lt.insert(lt.end(), T.phi_edge.begin(), T.phi_edge.end());
-  lt.insert(lt.end(), T.edge_w.begin(), T.edge_w.end());
-  lt.insert(lt.end(), T.cell_w.begin(), T.cell_w.end());
-  lt.insert(lt.end(), T.dref.begin(), T.dref.end());
-  if ((int)lt.size() != c->solver->limiter_table_len()) return fail(c, DGM_BAD_CONFIG, "limiter table size mismatch");
-  if (int s = upload(c, &S.ltab, lt)) return s;
-
-  // mesh, structure of arrays
-  const int64_t nel = M.ne, ned = M.nedge;
-  std::vector<int> el_edge(3 * (size_t)nes, 0), el_nbr(3 * (size_t)nes, -1);
-  std::vector<double> el_geo((size_t)G_N * nes, 0.0);
-  for (int64_t e = 0; e < nel; ++e) {
-    for (int i = 0; i < 3; ++i) {
-      const int64_t id = M.el_edge[3 * e + i];
-      const int side = (M.ed_elem[2 * id + 1] == e && M.ed_lic[2 * id + 1] == i) ? 1 : 0;
-      el_edge[i * nes + e] = (int)(id << 1) | side;
-      el_nbr[i * nes + e] = (int)M.el_nbr[3 * e + i];
-    }
-    const double J = M.el_J[e];
-    el_geo[G_J * nes + e] = J;
-    el_geo[G_XX * nes + e] = M.el_metric[4 * e + 0];
-    el_geo[G_XY * nes + e] = M.el_metric[4 * e + 1];
-    el_geo[G_EX * nes + e] = M.el_metric[4 * e + 2];
-    el_geo[G_EY * nes + e] = M.el_metric[4 * e + 3];
-    el_geo[G_CR * nes + e] = M.el_cr[e];
-    el_geo[G_HMIN * nes + e] = M.el_hmin[e];
-    el_geo[G_2J * nes + e] = 2.0 / J;
-  }
-  std::vector<int> ed_elem(2 * ned), ed_info(ned);
-  std::vector<double> ed_geo(3 * ned);
-  for (int64_t i = 0; i < ned; ++i) {
-    ed_elem[i] = (int)M.ed_elem[2 * i];
-    ed_elem[ned + i] = (int)M.ed_elem[2 * i + 1];
-    const int licL = (int)M.ed_lic[2 * i];
-    if (M.ed_patch[i] < 0) {
-      ed_info[i] = licL | ((int)M.ed_lic[2 * i + 1] << 2) | (BC_INTERIOR << 4);
-      ed_geo[i] = M.ed_normal[2 * i];
-      ed_geo[ned + i] = M.ed_normal[2 * i + 1];
-    } else {
-      const int patch = (int)M.ed_patch[i];
-      const int bc = M.patch_type[patch] == DGM_WALL ? BC_WALL : BC_FARFIELD;
-      ed_info[i] = licL | (bc << 4) | (patch << 8);
-      ed_geo[i] = M.ed_bc_normal[2 * i];
-      ed_geo[ned + i] = M.ed_bc_normal[2 * i + 1];
-    }
-    ed_geo[2 * ned + i] = 0.5 * M.ed_len[i];
-  }
-  // far-field conserved state per patch (src/initialization.rs:124-133, boundary_condition.rs:50-55)
   const dgm_params& P = c->prm;
-  const double cinf = std::sqrt(P.fs_hcr * P.fs_p / P.fs_rho);
-  const double rad = P.fs_aoa_deg * (M_PI / 180.0);
-  const double uinf = P.fs_mach * cinf * std::cos(rad), vinf = P.fs_mach * cinf * std::sin(rad);
-  std::vector<double> patch_q;
-  for (size_t b = 0; b < M.patch_type.size(); ++b) {
-    patch_q.push_back(P.fs_rho);
-    patch_q.push_back(P.fs_rho * uinf);
-    patch_q.push_back(P.fs_rho * vinf);
-    patch_q.push_back(P.fs_p / (P.hcr - 1.0) + 0.5 * P.fs_rho * (uinf * uinf + vinf * vinf));
-  }
-  DevMesh& dm = S.mesh;
-  dm.ne = (int)nel;
-  dm.nes = nes;
-  dm.nedge = (int)ned;
-  dm.npatch = (int)M.patch_type.size();
-  int* pi;
-  double* pd;
-  float* pf;
-  if (int s = upload(c, &pi, el_edge)) return s;
-  dm.el_edge = pi;
-  if (int s = upload(c, &pi, el_nbr)) return s;
-  dm.el_nbr = pi;
-  if (int s = upload(c, &pd, el_geo)) return s;
-  dm.el_geo = pd;
-  if (int s = upload(c, &pf, std::vector<float>(el_geo.begin(), el_geo.end()))) return s;
-  dm.el_geo32 = pf;
-  if (int s = upload(c, &pi, ed_elem)) return s;
-  dm.ed_elem = pi;
-  if (int s = upload(c, &pi, ed_info)) return s;
-  dm.ed_info = pi;
-  if (int s = upload(c, &pd, ed_geo)) return s;
-  dm.ed_geo = pd;
-  if (int s = upload(c, &pf, std::vector<float>(ed_geo.begin(), ed_geo.end()))) return s;
-  dm.ed_geo32 = pf;
-  if (int s = upload(c, &pd, patch_q)) return s;
-  dm.patch_q = pd;
-
-  // state and work buffers
-  const size_t ns = (size_t)4 * np * nes;
-  const size_t eb = f32 ? 4 : 8;
-  if (int s = dalloc(c, &S.un, ns)) return s;
-  HIPCHK(c, hipMemset(S.un, 0, ns * 8));
-  if (f32) {
-    float *a, *b2, *d;
-    if (int s = dalloc(c, &a, ns)) return s;
-    if (int s = dalloc(c, &b2, ns)) return s;
-    if (int s = dalloc(c, &d, ns)) return s;
-    HIPCHK(c, hipMemset(a, 0, ns * 4));
-    HIPCHK(c, hipMemset(b2, 0, ns * 4));
-    HIPCHK(c, hipMemset(d, 0, ns * 4));
-    S.sn = a;
-    S.s = b2;
-    S.d = d;
-  } else {
-    double* a;
-    if (int s = dalloc(c, &a, ns)) return s;
-    HIPCHK(c, hipMemset(a, 0, ns * 8));
-    S.us = a;
-  }
-  if (int s = dalloc(c, &S.rout, ns)) return s;
-  char* fl;
-  if (int s = dalloc(c, &fl, (size_t)ned * ne * 4 * eb)) return s;
-  S.flux = fl;
-  if (int s = dalloc(c, &S.lim_list, (size_t)nel)) return s;
-  if (int s = dalloc(c, &S.lim_w, (size_t)nel * 4 * np)) return s;
-  if (int s = dalloc(c, &S.lim_flags, (size_t)nel)) return s;
-  if (int s = dalloc(c, &S.ctl, 1)) return s;
-  if (int s = dalloc(c, &c->hbuf, nstate(c))) return s;
+  std::string e;
+  if (int s = dalloc(c, &c->d_case, 1)) return s;
+  if (int s = dalloc(c, &c->d_dtmin, DT_CASE)) return s;
   if (int s = dalloc(c, &c->d_force, 2)) return s;
   if (int s = dalloc(c, &c->d_flag, 1)) return s;
-  std::vector<int> lv(M.level_elems.begin(), M.level_elems.end());
-  if (int s = upload(c, &c->d_levels, lv)) return s;
-  S.levels = c->d_levels;
-  S.level_offsets.assign(M.level_offsets.begin(), M.level_offsets.end());
-  S.limiter = c->prm.limiter;
-  S.hcr = c->prm.hcr;
+  c->A.reset(new Arena());
+  Arena& A = *c->A;
+  if (int s = A.init(P.order, P.n_cell_gp, P.n_edge_gp, P.precision, P.limiter, c->M.ne, c->M.nedge,
+                     (int64_t)c->M.patch_type.size(), true, c->d_case, c->d_dtmin, c->stream, e))
+    return fail(c, s, e);
+  A.S.mesh.hcr = P.hcr;
   {
     const char* v = std::getenv("DGM_ELEMENT_KERNEL");  // "valu" selects the one-lane-per-element kernel
-    S.use_mfma = (v && std::strcmp(v, "valu") == 0) ? 0 : 1;
-    const char* r = std::getenv("DGM_DETECT_REMAP");
-    S.remap_detect = (r && r[0] == '0') ? 0 : 1;
+    if (v && std::strcmp(v, "valu") == 0) A.S.use_mfma = 0;
+    const char* r = std::getenv("DGM_REMAP");            // "0": plain block order in the gather kernels
+    if (r && r[0] == '0') A.S.tune.remap = 0;
   }
-  if (c->prm.limiter != DGM_LIMITER_OFF) {
-    if (int s = dalloc(c, &S.smat, (size_t)nel * c->solver->smoothness_len())) return s;
-    c->solver->build_smoothness(S, c->stream);
-    HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-  }
-  S.weno_grid = 2048;
-  S.apply_grid = 1024;
-  return write_ctl(c, fresh_ctl(c));
+  // the case record first (the smoothness build and the kernels read it), then the mesh
+  if (int s = write_ctl(c, fresh_ctl(c))) return s;
+  const std::vector<double> pq = farfield_states(c->M, P.fs_mach, P.fs_aoa_deg, P.fs_rho, P.fs_p, P.fs_hcr, P.hcr);
+  if (int s = A.place(c->M, pq, 0, c->pl, e)) return fail(c, s, e.empty() ? "mesh does not fit its arena" : e);
+  A.update_extent();
+  std::vector<int> lv(c->M.level_elems.begin(), c->M.level_elems.end());
+  if (int s = dalloc(c, &c->d_levels, lv.size())) return s;
+  if (!lv.empty()) HIPCHK(c, hipMemcpy(c->d_levels, lv.data(), lv.size() * 4, hipMemcpyHostToDevice));
+  A.S.levels = c->d_levels;
+  A.S.level_offsets.assign(c->M.level_offsets.begin(), c->M.level_offsets.end());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DGM_OK;
 }
 
 int create(const dgm_mesh* mesh, const dgm_params* params, dgm_ctx** out, bool device) {
@@ -363,11 +207,10 @@ int create(const dgm_mesh* mesh, const dgm_params* params, dgm_ctx** out, bool d
   if (P.limiter < DGM_LIMITER_OFF || P.limiter > DGM_LIMITER_JACOBI) return DGM_BAD_CONFIG;
   if (!(P.hcr > 1.0)) return DGM_BAD_CONFIG;
   if (!build_tables(c->T, P.order, P.n_cell_gp, P.n_edge_gp)) return DGM_BAD_CONFIG;
-  c->solver = make_solver(P.precision, P.order, P.n_cell_gp, P.n_edge_gp);
-  if (!c->solver) return DGM_BAD_CONFIG;
+  if (!make_solver(P.precision, P.order, P.n_cell_gp, P.n_edge_gp)) return DGM_BAD_CONFIG;
   std::string err;
   if (int s = build_mesh(*mesh, c->M, err)) return s;
-  if (c->M.ne >= (1 << 30) || c->M.nedge >= (1 << 29)) return DGM_BAD_MESH;
+  if (c->M.ne >= (1 << 29) || c->M.nedge >= (1 << 28)) return DGM_BAD_MESH;
   if (P.cfl > 0.0) {
     c->cfl = P.cfl;
   } else {
@@ -406,6 +249,7 @@ void dgm_ctx_destroy(dgm_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->A.reset();
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -426,7 +270,7 @@ int dgm_ctx_info(const dgm_ctx* c, dgm_info* info) {
   info->limiter = c->prm.limiter;
   info->n_limiter_levels = (int32_t)c->M.level_offsets.size() - 1;
   info->cfl = c->cfl;
-  info->device_bytes = c->bytes;
+  info->device_bytes = c->A ? c->A->bytes : 0;
   return DGM_OK;
 }
 
@@ -557,8 +401,8 @@ int dgm_state_upload(dgm_ctx* c, const double* U) {
 }
 
 int dgm_state_download(dgm_ctx* c, double* U) {
-  if (!c || !U) return DGM_BAD_ARGUMENT;
-  return stage_out(c, c->S.un, U);
+  if (!c || !U || !c->stream) return DGM_BAD_ARGUMENT;
+  return stage_out(c, dev(c).un, U);
 }
 
 int dgm_compute_residuals(dgm_ctx* c, double* R, const double* U, int32_t accumulate) {
@@ -566,11 +410,11 @@ int dgm_compute_residuals(dgm_ctx* c, double* R, const double* U, int32_t accumu
   std::vector<double> rin;
   if (accumulate) rin.assign(R, R + nstate(c));
   if (int s = stage_in(c, U)) return s;
-  Ctl h = fresh_ctl(c);
+  CaseCtl h = fresh_ctl(c);
   if (int s = write_ctl(c, h)) return s;
-  c->solver->residual(c->S, c->stream);
+  solver(c).residual(dev(c), c->stream);
   HIPCHK(c, hipGetLastError());
-  if (int s = stage_out(c, c->S.rout, R)) return s;
+  if (int s = stage_out(c, dev(c).rout, R)) return s;
   if (int s = read_ctl(c, h)) return s;
   if (int s = ctl_status(c, h)) return s;
   if (accumulate) {
@@ -589,12 +433,15 @@ int dgm_apply_limiter(dgm_ctx* c, double* U, uint8_t* troubled) {
   if (!c || !U) return DGM_BAD_ARGUMENT;
   if (int s = stage_in(c, U)) return s;
   if (int s = write_ctl(c, fresh_ctl(c))) return s;
-  if (troubled) HIPCHK(c, hipMemsetAsync(c->S.lim_flags, 0, c->M.ne, c->stream));
-  c->solver->limiter(c->S, c->stream);
+  if (troubled) HIPCHK(c, hipMemsetAsync(dev(c).lim_flags, 0, c->M.ne, c->stream));
+  solver(c).limiter(dev(c), c->stream);
   HIPCHK(c, hipGetLastError());
-  if (int s = stage_out(c, c->S.un, U)) return s;
-  if (troubled) HIPCHK(c, hipMemcpy(troubled, c->S.lim_flags, c->M.ne, hipMemcpyDeviceToHost));
-  Ctl h;
+  if (int s = stage_out(c, dev(c).un, U)) return s;
+  if (troubled) {
+    HIPCHK(c, hipMemcpyAsync(troubled, dev(c).lim_flags, c->M.ne, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  CaseCtl h;
   if (int s = read_ctl(c, h)) return s;
   return ctl_status(c, h);
 }
@@ -602,15 +449,17 @@ int dgm_apply_limiter(dgm_ctx* c, double* U, uint8_t* troubled) {
 int dgm_compute_time_step(dgm_ctx* c, const double* U, double* dt) {
   if (!c || !U || !dt) return DGM_BAD_ARGUMENT;
   if (int s = stage_in(c, U)) return s;
-  Ctl h = fresh_ctl(c);
+  CaseCtl h = fresh_ctl(c);
   if (int s = write_ctl(c, h)) return s;
-  c->solver->time_step(c->S, c->stream);
+  solver(c).time_step(dev(c), c->stream, 1);
   HIPCHK(c, hipGetLastError());
-  if (int s = read_ctl(c, h)) return s;
+  std::vector<unsigned long long> slot(DT_SLOT);
+  HIPCHK(c, hipMemcpyAsync(slot.data(), c->d_dtmin, DT_SLOT * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   double m = 1.0e10;
   for (int i = 0; i < DT_LANES; ++i) {
     double v;
-    std::memcpy(&v, &h.dtmin_bits[0][i * DT_STRIDE], 8);
+    std::memcpy(&v, &slot[i * DT_STRIDE], 8);
     m = std::fmin(m, v);
   }
   *dt = m;
@@ -620,14 +469,14 @@ int dgm_compute_time_step(dgm_ctx* c, const double* U, double* dt) {
 int dgm_rk3(dgm_ctx* c, double* U, double dt) {
   if (!c || !U) return DGM_BAD_ARGUMENT;
   if (int s = stage_in(c, U)) return s;
-  Ctl h = fresh_ctl(c);
+  CaseCtl h = fresh_ctl(c);
   h.dt = dt;
   if (int s = write_ctl(c, h)) return s;
-  for (int st = 0; st < 3; ++st) c->solver->stage(c->S, st, c->stream);
+  for (int st = 0; st < 3; ++st) solver(c).stage(dev(c), st, c->stream);
   HIPCHK(c, hipGetLastError());
   if (int s = read_ctl(c, h)) return s;
   if (int s = ctl_status(c, h)) return s;
-  return stage_out(c, c->S.un, U);
+  return stage_out(c, dev(c).un, U);
 }
 
 int dgm_device_time_march(dgm_ctx* c, int32_t final_step, double final_time, dgm_march_stats* stats) {
@@ -635,7 +484,7 @@ int dgm_device_time_march(dgm_ctx* c, int32_t final_step, double final_time, dgm
   if (int s = ensure_graph(c)) return s;
   if (int s = start_run(c, final_step, final_time)) return s;
   const int batch = 16;
-  Ctl h;
+  CaseCtl h;
   double t = 0.0;
   int32_t steps = 0;
   while (true) {
@@ -657,7 +506,7 @@ int dgm_time_march(dgm_ctx* c, double* U, int32_t final_step, double final_time,
   if (!c || !U) return DGM_BAD_ARGUMENT;
   if (int s = stage_in(c, U)) return s;
   const int st = dgm_device_time_march(c, final_step, final_time, stats);
-  if (int s = stage_out(c, c->S.un, U)) return s;
+  if (int s = stage_out(c, dev(c).un, U)) return s;
   return st;
 }
 
@@ -671,7 +520,7 @@ int dgm_device_steps_async(dgm_ctx* c, int32_t n) {
 int dgm_synchronize(dgm_ctx* c) {
   if (!c) return DGM_BAD_ARGUMENT;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  Ctl h;
+  CaseCtl h;
   if (int s = read_ctl(c, h)) return s;
   return ctl_status(c, h);
 }
@@ -681,7 +530,7 @@ void* dgm_stream(dgm_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int dgm_check_state(dgm_ctx* c) {
   if (!c || !c->stream) return DGM_BAD_ARGUMENT;
   HIPCHK(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
-  c->solver->check_finite(c->S, c->d_flag, c->stream);
+  solver(c).check_finite(dev(c), c->d_flag, 0, (int)c->M.ne, c->stream);
   int f = 0;
   HIPCHK(c, hipMemcpyAsync(&f, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -690,7 +539,7 @@ int dgm_check_state(dgm_ctx* c) {
 
 int dgm_device_clock(dgm_ctx* c, double* t, int32_t* steps, double* last_dt) {
   if (!c) return DGM_BAD_ARGUMENT;
-  Ctl h;
+  CaseCtl h;
   if (int s = read_ctl(c, h)) return s;
   double te;
   int32_t se;
@@ -712,13 +561,13 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   for (int it = 0; it < n; ++it) {
     int k = 0;
     HIPCHK(c, hipEventRecord(ev[k++], c->stream));
-    if (!fused_dt(c)) c->solver->time_step(c->S, c->stream);
+    if (!fused_dt(c)) solver(c).time_step(dev(c), c->stream, 0);
     HIPCHK(c, hipEventRecord(ev[k++], c->stream));
-    launch_step_begin(c->S.ctl, c->stream);
+    launch_step_begin(dev(c).ctl, 1, c->stream);
     HIPCHK(c, hipEventRecord(ev[k++], c->stream));
     for (int st = 0; st < 3; ++st)
       for (int part = 0; part < Solver::N_PARTS; ++part) {
-        c->solver->stage_part(c->S, st, part, c->stream);
+        solver(c).stage_part(dev(c), st, part, c->stream);
         HIPCHK(c, hipEventRecord(ev[k++], c->stream));
       }
     HIPCHK(c, hipEventSynchronize(ev[k - 1]));
@@ -746,14 +595,14 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   out->apply_ms = acc[4] / (3.0 * n);
   out->step_ms = t_step / n;
   out->steps = n;
-  Ctl h;
+  CaseCtl h;
   if (int s = read_ctl(c, h)) return s;
   return ctl_status(c, h);
 }
 
 int dgm_troubled_counts(dgm_ctx* c, int32_t counts[4]) {
   if (!c || !counts) return DGM_BAD_ARGUMENT;
-  Ctl h;
+  CaseCtl h;
   if (int s = read_ctl(c, h)) return s;
   for (int i = 0; i < 4; ++i) counts[i] = h.count[i];
   return DGM_OK;
@@ -762,7 +611,7 @@ int dgm_troubled_counts(dgm_ctx* c, int32_t counts[4]) {
 int dgm_lift_coefficient(dgm_ctx* c, double chord, double* cl) {
   if (!c || !c->stream || !cl || !(chord > 0.0)) return DGM_BAD_ARGUMENT;
   HIPCHK(c, hipMemsetAsync(c->d_force, 0, 2 * sizeof(double), c->stream));
-  c->solver->wall_force(c->S, c->d_force, c->stream);
+  solver(c).wall_force(dev(c), c->d_force, 0, (int)c->M.nedge, c->stream);
   double f[2];
   HIPCHK(c, hipMemcpyAsync(f, c->d_force, sizeof(f), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

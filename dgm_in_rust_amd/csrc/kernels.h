@@ -2,11 +2,16 @@
 //
 // Device layout (structure of arrays, element index fastest so that one lane per element
 // gives coalesced loads):
-//   state   [4 * Np][nes]  component c = var * Np + mode, nes = n_elements rounded up to 64
+//   state   [4 * Np][nes]  component c = var * Np + mode, nes = element capacity (a multiple of 64)
 //   flux    [n_edges][Ne][4]  numerical flux times half the edge length at each edge point
 //   element [3][nes] edge id << 1 | side (side 1 = the element is the edge's right neighbour),
 //           [3][nes] neighbour id or -1, geometry [8][nes]
 //   edge    [2][n_edges] left / right element, info word, geometry [3][n_edges]
+//
+// One arena holds any number of cases of one polynomial order as disconnected pieces of one mesh.
+// Every case's element range and edge range start at a multiple of 64, so each 64-wide wavefront
+// of a one-lane-per-element (or per-edge) kernel belongs to exactly one case; el_case64[e >> 6] and
+// ed_case64[ed >> 6] give it (-1: unused capacity). A single-case context is an arena with one case.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,32 +25,52 @@ enum EdgeBc { BC_INTERIOR = 0, BC_WALL = 1, BC_FARFIELD = 2 };
 enum ElemOp { OP_RESID = 0, OP_S0 = 1, OP_S1 = 2, OP_S2 = 3 };
 
 struct DevMesh {
-  int ne, nes, nedge, npatch;
+  int ne, nes, nedge, npatch;  // ne / nedge: extent the kernels cover; nes: element capacity (stride)
+  double hcr;
   const int* el_edge;      // [3][nes]
   const int* el_nbr;       // [3][nes]
   const double* el_geo;    // [G_N][nes]
   const float* el_geo32;   // [G_N][nes]
-  const int* ed_elem;      // [2][nedge]
+  const int* ed_elem;      // [2][nedge capacity]
   const int* ed_info;      // [nedge]: lic_left | lic_right << 2 | bc << 4 | patch << 8
-  const double* ed_geo;    // [3][nedge]: nx, ny, half length (boundary edges: normal oriented by the owner)
-  const float* ed_geo32;   // [3][nedge]
+  const double* ed_geo;    // [3][nedge capacity]: nx, ny, half length (boundary edges: normal oriented by the owner)
+  const float* ed_geo32;   // [3][nedge capacity]
   const double* patch_q;   // [npatch][4] far-field conserved state
+  const int* el_case64;    // [nes / 64] case of each 64-element block, -1 unused
+  const int* ed_case64;    // [edge capacity / 64] case of each 64-edge block, -1 unused
+  int ed_stride;           // edge capacity (stride of ed_elem / ed_geo)
+  int pad;
 };
 
-// Step control, resident on the device so that whole steps replay from a captured graph.
+// Step control of one case, resident on the device so that whole steps replay from a captured graph.
 // A step is counted at the start of the next one (pending): the time-step reduction for step n + 1
-// is fused into step n's last limiter pass and lands in dtmin_bits[(n + 1) & 1].
-// The reduction targets are spread over DT_LANES words 128 bytes apart: atomics from many workgroups
-// to one word serialise in L2, so each workgroup / wavefront picks a word by its index.
-constexpr int DT_LANES = 32;
-constexpr int DT_STRIDE = 16;  // unsigned long longs between two targets (128 bytes)
-struct Ctl {
-  double t, dt, final_time, cfl, hcr;
-  unsigned long long error;         // (edge << 8) | status of the lowest failing edge, ~0ull when clean
+// is fused into step n's last limiter pass and lands in the case's dt slot (n + 1) & 1.
+struct CaseCtl {
+  double t, dt, final_time, cfl;
+  unsigned long long error;         // (local edge << 8) | status of the lowest failing edge, ~0ull when clean
   int step, final_step, active, pending;
-  int count[4];                     // troubled cells per stage (slot 3: stand-alone limiter call)
-  int last_count, pad;
-  unsigned long long dtmin_bits[2][DT_LANES * DT_STRIDE];  // atomicMin targets by step parity
+  int live;                         // the slot holds a case
+  int e_off, ed_off;                // first element / edge of the case in its arena
+  int e_end, ed_end;                // one past its last element / edge (its 64-blocks may extend further)
+  int group;                        // arena (order) of the case
+  int count[4];                     // troubled cells per stage of the running step (slot 3: stand-alone limiter)
+  int last_count;                   // troubled cells of the last stage of the previous step
+  int pad[7];
+};
+static_assert(sizeof(CaseCtl) == 128, "CaseCtl is one 128-byte record");
+
+// The time-step minimum of a case is reduced into DT_LANES words 128 bytes apart: atomics from many
+// workgroups to one word serialise in L2, so each wavefront picks a word by its index.
+constexpr int DT_LANES = 32;
+constexpr int DT_STRIDE = 16;                       // unsigned long longs between two targets (128 bytes)
+constexpr int DT_SLOT = DT_LANES * DT_STRIDE;       // one slot
+constexpr int DT_CASE = 2 * DT_SLOT;                // two slots (step parity) per case
+
+// Kernel-argument view of the control state of one arena.
+struct Ctl {
+  CaseCtl* cases;                // [max cases]
+  unsigned long long* dtmin;     // [max cases][2][DT_SLOT]
+  int* count;                    // [4] troubled-list lengths of this arena per stage slot
 };
 
 // Buffers the element kernel reads and writes. FP64 mode uses in/un/us; FP32-bulk uses the
@@ -65,6 +90,16 @@ struct LimiterBufs {
   int* list;       // troubled element ids
   double* w;       // reconstructed polynomials [slot][4][Np]
   uint8_t* flags;  // optional per-element troubled flags
+};
+
+// Launch shapes of one arena, chosen by the autotuner (defaults are the measured best at p = 3).
+struct Tuning {
+  int surface_block = 256;   // threads per block of the edge kernel (64, 128, 256)
+  int element_waves = 4;     // FP32 matrix-core element kernel: wavefronts (16 elements each) per block (1, 2, 4)
+  int element_block = 128;   // FP64 element kernel: threads per block (64, 128, 256)
+  int detect_block = 256;    // detector threads per block (64, 128, 256)
+  int remap = 1;             // XCD-contiguous block order for the gather kernels
+  int weno_grid = 2048;      // wavefronts of the WENO rebuild
 };
 
 }  // namespace dgm

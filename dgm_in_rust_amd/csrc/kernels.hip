@@ -236,8 +236,15 @@ __device__ __forceinline__ void farfield_flux(const T lv[4], const double* qinf,
   out[3] = (q[3] + pn) * vnn;
 }
 
-__device__ __forceinline__ void record_error(Ctl* ctl, int edge, int code) {
-  atomicMin(&ctl->error, ((unsigned long long)(unsigned)edge << 8) | (unsigned long long)code);
+__device__ __forceinline__ void record_error(CaseCtl* cc, int edge, int code) {
+  atomicMin(&cc->error, ((unsigned long long)(unsigned)edge << 8) | (unsigned long long)code);
+}
+
+__device__ __forceinline__ bool case_runs(const CaseCtl& cc) { return cc.active && cc.error == ~0ull; }
+
+// dt slot of a case for the given step parity
+__device__ __forceinline__ unsigned long long* dt_slot(const Ctl& ctl, int c, int parity) {
+  return ctl.dtmin + (size_t)c * DT_CASE + (size_t)(parity & 1) * DT_SLOT;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -245,21 +252,26 @@ __device__ __forceinline__ void record_error(Ctl* ctl, int edge, int code) {
 // ---------------------------------------------------------------------------------------
 template <class T, int NP, int NQ, int NE>
 __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict__ U, T* __restrict__ G,
-                                                 const T* __restrict__ tab, Ctl* __restrict__ ctl, int force) {
+                                                 const T* __restrict__ tab, Ctl ctl, int force, int remap) {
   using O = TabOff<NP, NQ, NE>;
   __shared__ T sphi[3 * NE * NP];
   for (int i = threadIdx.x; i < 3 * NE * NP; i += blockDim.x) sphi[i] = tab[O::PHIE + i];
   __syncthreads();
-  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
-  const int ed = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int ed = blk * blockDim.x + threadIdx.x;
   if (ed >= m.nedge) return;
-  const int nes = m.nes;
+  const int cs = m.ed_case64[ed >> 6];
+  if (cs < 0) return;
+  CaseCtl* cc = ctl.cases + cs;
+  if (!force && !case_runs(*cc)) return;
+  if (ed >= cc->ed_end) return;
+  const int nes = m.nes, est = m.ed_stride;
   const int info = m.ed_info[ed];
   const int licL = info & 3, licR = (info >> 2) & 3, bc = (info >> 4) & 3, patch = info >> 8;
   const int L = m.ed_elem[ed];
   const T* eg = geo_ed<T>(m);
-  const T nx = eg[ed], ny = eg[m.nedge + ed], hl = eg[2 * m.nedge + ed];
-  const T hcr = (T)ctl->hcr;
+  const T nx = eg[ed], ny = eg[est + ed], hl = eg[2 * est + ed];
+  const T hcr = (T)m.hcr;
   T ul[NE][4];
 #pragma unroll
   for (int q = 0; q < NE; ++q)
@@ -275,7 +287,7 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
     }
   T* g = G + (size_t)ed * NE * 4;
   if (bc == BC_INTERIOR) {
-    const int R = m.ed_elem[m.nedge + ed];
+    const int R = m.ed_elem[est + ed];
     T ur[NE][4];
 #pragma unroll
     for (int q = 0; q < NE; ++q)
@@ -298,7 +310,7 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
 #pragma unroll
       for (int v = 0; v < 4; ++v) g[q * 4 + v] = f[v] * hl;
     }
-    if (err) record_error(ctl, ed, err);
+    if (err) record_error(cc, ed - cc->ed_off, err);
   } else if (bc == BC_WALL) {
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
@@ -324,17 +336,19 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
 // element kernel: one lane per element — volume integral + flux lift + mass inverse + RK stage
 // ---------------------------------------------------------------------------------------
 template <class T, int NP, int NQ, int NE, int OP>
-__global__ __launch_bounds__(128) void k_element(DevMesh m, StageBufs<T> b, const T* __restrict__ G,
-                                                 const T* __restrict__ tab, Ctl* __restrict__ ctl, int force) {
+__global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, const T* __restrict__ G,
+                                                 const T* __restrict__ tab, Ctl ctl, int force, int remap) {
   using O = TabOff<NP, NQ, NE>;
-  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
-  const int e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (e == 0) {
-    // the limiter of this stage counts troubled cells from zero
-    const int slot = OP == OP_S0 ? 0 : (OP == OP_S1 ? 1 : 2);
-    ctl->count[slot] = 0;
-  }
+  // the limiter pass of this stage builds its troubled list from zero
+  if (OP != OP_RESID && blockIdx.x == 0 && threadIdx.x == 0) ctl.count[OP - OP_S0] = 0;
+  const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int e = blk * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
+  const int cs = m.el_case64[e >> 6];
+  if (cs < 0) return;
+  const CaseCtl& cc = ctl.cases[cs];
+  if (!force && !case_runs(cc)) return;
+  if (e >= cc.e_end) return;
   const int nes = m.nes;
   T u[4][NP];
 #pragma unroll
@@ -344,7 +358,7 @@ __global__ __launch_bounds__(128) void k_element(DevMesh m, StageBufs<T> b, cons
   const T* eg = geo_el<T>(m);
   const T xx = eg[G_XX * nes + e], xy = eg[G_XY * nes + e], ex = eg[G_EX * nes + e], ey = eg[G_EY * nes + e];
   const T j2 = eg[G_2J * nes + e];
-  const T hcr = (T)ctl->hcr;
+  const T hcr = (T)m.hcr;
   T r[4][NP];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -398,7 +412,7 @@ __global__ __launch_bounds__(128) void k_element(DevMesh m, StageBufs<T> b, cons
       for (int k = 0; k < NP; ++k) b.rout[(size_t)(v * NP + k) * nes + base] = (double)r[v][k];
     return;
   }
-  const double dt = ctl->dt;
+  const double dt = cc.dt;
   if constexpr (sizeof(T) == 8) {
     // src/temporal_disc/rk3.rs:10,16,22
 #pragma unroll
@@ -466,20 +480,22 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 
 template <int NP, int NQ, int NE, int OP>
 __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float> b, const float* __restrict__ G,
-                                                      const float* __restrict__ tab, Ctl* __restrict__ ctl, int force) {
+                                                      const float* __restrict__ tab, Ctl ctl, int force, int remap) {
   static_assert(NP <= 16 && NQ <= 16 && 3 * NE <= 16, "matrix-core element kernel needs p <= 3");
   using O = TabOff<NP, NQ, NE>;
-  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
-  const int tid = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (tid == 0) {
-    const int slot = OP == OP_S0 ? 0 : (OP == OP_S1 ? 1 : 2);
-    ctl->count[slot] = 0;
-  }
+  if (OP != OP_RESID && blockIdx.x == 0 && threadIdx.x == 0) ctl.count[OP - OP_S0] = 0;
+  const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int tid = blk * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const int wave = tid >> 6;
+  // the wavefront's 16 elements lie in one 64-element block, hence in one case (uniform exit)
+  const int cs = m.el_case64[(wave * 16) >> 6];
+  if (cs < 0) return;
+  const CaseCtl& cc = ctl.cases[cs];
+  if (!force && !case_runs(cc)) return;
   const int c = lane & 15, g = lane >> 4;
   const int e = wave * 16 + c;
-  const bool valid = e < m.ne;
+  const bool valid = e < m.ne && e < cc.e_end;
   const int nes = m.nes;
   const int es = valid ? e : 0;
   // per-lane table operands (A of each product)
@@ -514,7 +530,7 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
   const float* eg = m.el_geo32;
   const float xx = eg[G_XX * nes + es], xy = eg[G_XY * nes + es], ex = eg[G_EX * nes + es], ey = eg[G_EY * nes + es];
   const float j2 = eg[G_2J * nes + es];
-  const float hcr = (float)ctl->hcr;
+  const float hcr = (float)m.hcr;
   // fluxes at the lane's four points, in the metric-transformed form
   float ft[4][4], gt[4][4];  // [var][r]
 #pragma unroll
@@ -575,7 +591,7 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
       }
     return;
   }
-  const double dt = ctl->dt;
+  const double dt = cc.dt;
   const float dtf = (float)dt;
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -647,43 +663,54 @@ __device__ __forceinline__ double read_dt_slot(const unsigned long long* slot_ba
 }
 
 // stand-alone time-step reduction into the slot of the step about to begin
+// (one wavefront = one 64-element block = one case; every lane of it reaches the reduction)
 template <class T, int NP>
-__global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, Ctl* __restrict__ ctl) {
+__global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, Ctl ctl, int force) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((e >> 6) >= (m.nes >> 6)) return;
+  const int cs = m.el_case64[e >> 6];
+  if (cs < 0) return;
+  const CaseCtl& cc = ctl.cases[cs];
+  if (!force && !cc.live) return;
   double dt = 1.0e10;
-  if (e < m.ne) {
+  if (e < m.ne && e < cc.e_end) {
     const int nes = m.nes;
-    const double d = element_dt<NP>([&](int v, int k) { return (double)U[(size_t)(v * NP + k) * nes + e]; }, ctl->hcr,
-                                    ctl->cfl, m.el_geo[G_HMIN * nes + e]);
+    const double d = element_dt<NP>([&](int v, int k) { return (double)U[(size_t)(v * NP + k) * nes + e]; }, m.hcr,
+                                    cc.cfl, m.el_geo[G_HMIN * nes + e]);
     if (d < dt) dt = d;
   }
-  wave_min_dt(dt, ctl->dtmin_bits[(ctl->step + ctl->pending) & 1], (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  wave_min_dt(dt, dt_slot(ctl, cs, cc.step + cc.pending), e >> 6);
 }
 
 // Step control (src/temporal_disc.rs:22-34): count the step that just ran, take its successor's dt
 // (clamped to final_time), and decide whether it runs (step < final_step && t < final_time).
-__global__ void k_step_begin(Ctl* ctl) {
-  if (ctl->pending) {
-    ctl->t += ctl->dt;
-    ctl->step += 1;
-    ctl->pending = 0;
-    ctl->last_count = ctl->count[2];
+// One thread per case slot.
+__global__ void k_step_begin(Ctl ctl, int ncases) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncases) return;
+  CaseCtl& c = ctl.cases[i];
+  if (!c.live) return;
+  if (c.pending) {
+    c.t += c.dt;
+    c.step += 1;
+    c.pending = 0;
+    c.last_count = c.count[2];
   }
-  const int slot = ctl->step & 1;
-  const double dtm = read_dt_slot(ctl->dtmin_bits[slot]);
-  for (int i = 0; i < DT_LANES; ++i)
-    ctl->dtmin_bits[slot][i * DT_STRIDE] = (unsigned long long)__double_as_longlong(1.0e10);
-  const bool go = ctl->step < ctl->final_step && ctl->t < ctl->final_time && ctl->error == ~0ull;
-  ctl->active = go ? 1 : 0;
+  unsigned long long* slot = dt_slot(ctl, i, c.step);
+  const double dtm = read_dt_slot(slot);
+  for (int k = 0; k < DT_LANES; ++k) slot[k * DT_STRIDE] = (unsigned long long)__double_as_longlong(1.0e10);
+  const bool go = c.step < c.final_step && c.t < c.final_time && c.error == ~0ull;
+  c.active = go ? 1 : 0;
   if (go) {
     double dt = dtm;
-    if (ctl->t + dt > ctl->final_time) dt = ctl->final_time - ctl->t;
-    ctl->dt = dt;
-    ctl->pending = 1;
+    if (c.t + dt > c.final_time) dt = c.final_time - c.t;
+    c.dt = dt;
+    c.pending = 1;
+    c.count[0] = c.count[1] = c.count[2] = 0;
   }
 }
 
-__global__ void k_reset_count(Ctl* ctl, int slot) { ctl->count[slot] = 0; }
+__global__ void k_reset_count(Ctl ctl, int slot) { ctl.count[slot] = 0; }
 
 // ---------------------------------------------------------------------------------------
 // limiter
@@ -730,8 +757,8 @@ __device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* _
     const int es = m.el_edge[lic * nes + e];
     const int edge = es >> 1;
     const double sgn = (es & 1) ? -1.0 : 1.0;
-    const double nx = sgn * m.ed_geo[edge], ny = sgn * m.ed_geo[m.nedge + edge];
-    const double len = 2.0 * m.ed_geo[2 * m.nedge + edge];
+    const double nx = sgn * m.ed_geo[edge], ny = sgn * m.ed_geo[m.ed_stride + edge];
+    const double len = 2.0 * m.ed_geo[2 * m.ed_stride + edge];
     double tr[NE], te[NE];
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
@@ -781,39 +808,46 @@ __device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* _
 
 // KXRCF detector, one lane per element. With fuse_dt (the last stage of a step) the next step's time
 // step is reduced here for the cells that stay untouched; k_weno adds the rebuilt ones.
+// (one wavefront = one 64-element block = one case: the exits below are uniform per wavefront)
 template <class T, int P, int NQ, int NE>
 __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ U, const double* __restrict__ lt,
-                                               LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force, int fuse_dt,
+                                               LimiterBufs lb, Ctl ctl, int slot, int force, int fuse_dt,
                                                int remap) {
-  constexpr int NP = np_of(P);
-  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
   const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int e = blk * blockDim.x + threadIdx.x;
+  if ((e >> 6) >= (m.nes >> 6)) return;
+  const int cs = m.el_case64[e >> 6];
+  if (cs < 0) return;
+  CaseCtl* cc = ctl.cases + cs;
+  if (!force && !case_runs(*cc)) return;
+  const bool valid = e < m.ne && e < cc->e_end;
   if (!fuse_dt) {
-    if (e >= m.ne) return;
+    if (!valid) return;
     const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e);
     if (lb.flags) lb.flags[e] = t ? 1 : 0;
     if (t) {
-      const int i = atomicAdd(&ctl->count[slot], 1);
+      const int i = atomicAdd(&ctl.count[slot], 1);
       lb.list[i] = e;
+      atomicAdd(&cc->count[slot], 1);
     }
     return;
   }
   double dt = 1.0e10;
-  if (e < m.ne) {
+  if (valid) {
     double sums[4];
     const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e, sums);
     if (lb.flags) lb.flags[e] = t ? 1 : 0;
     if (t) {
-      const int i = atomicAdd(&ctl->count[slot], 1);
+      const int i = atomicAdd(&ctl.count[slot], 1);
       lb.list[i] = e;
+      atomicAdd(&cc->count[slot], 1);
     } else {
-      const double d = element_dt<1>([&](int v, int) { return sums[v]; }, ctl->hcr, ctl->cfl,
+      const double d = element_dt<1>([&](int v, int) { return sums[v]; }, m.hcr, cc->cfl,
                                      m.el_geo[G_HMIN * m.nes + e]);
       if (d < dt) dt = d;
     }
   }
-  wave_min_dt(dt, ctl->dtmin_bits[(ctl->step + 1) & 1], (blk * (int)blockDim.x + (int)threadIdx.x) >> 6);
+  wave_min_dt(dt, dt_slot(ctl, cs, cc->step + 1), e >> 6);
 }
 
 // Smoothness matrix of one element: beta(P) = sum_{l=1..p} J^(l-1) sum_{l1=0..l} sum_q (J/2) w_q
@@ -830,14 +864,15 @@ struct SmoothShared {
 };
 
 template <int P, int NQ, int NE>
-__global__ __launch_bounds__(64) void k_smoothness(DevMesh m, const double* __restrict__ lt, double* __restrict__ smat) {
+__global__ __launch_bounds__(64) void k_smoothness(DevMesh m, const double* __restrict__ lt, double* __restrict__ smat,
+                                                  int e0) {
   using LO = LimOff<P, NQ, NE>;
   constexpr int NP = LO::NP;
   constexpr int NKW = LO::NKW;
   constexpr int NPK = NP * (NP + 1) / 2;
   __shared__ SmoothShared<P, NQ, NE> sh;
-  const int e = blockIdx.x;
-  if (e >= m.ne) return;
+  const int e = e0 + (int)blockIdx.x;
+  if (e >= m.nes) return;
   const int lane = threadIdx.x;
   const int nes = m.nes;
   const double J = m.el_geo[G_J * nes + e];
@@ -955,7 +990,7 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
 #pragma unroll
     for (int k = 0; k < NP; ++k) Pk[k] = 0.0;
     if (sh.nb[i] >= 0) {
-      const double nx = m.ed_geo[sh.edge[i]], ny = m.ed_geo[m.nedge + sh.edge[i]];
+      const double nx = m.ed_geo[sh.edge[i]], ny = m.ed_geo[m.ed_stride + sh.edge[i]];
       const double mean[4] = {sh.src[0][0][0], sh.src[0][1][0], sh.src[0][2][0], sh.src[0][3][0]};
       double Lm[16], Rm[16];
       eigenmatrix(mean, nx, ny, hcr, Lm, Rm);
@@ -1058,24 +1093,26 @@ __device__ __forceinline__ void store_limited(const StageBufs<T>& b, int nes, in
   }
 }
 
+// The troubled list holds elements of running cases only (the detector skips the others).
 template <class T, int P, int NQ, int NE>
 __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U, const double* __restrict__ smat,
-                                             LimiterBufs lb, Ctl* __restrict__ ctl, int slot, int force, int fuse_dt) {
+                                             LimiterBufs lb, Ctl ctl, int slot, int fuse_dt) {
   __shared__ WenoShared<P, NQ, NE> sh;
-  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
   constexpr int NP = np_of(P);
-  const int n = ctl->count[slot];
+  const int n = ctl.count[slot];
   for (int s = blockIdx.x; s < n; s += gridDim.x) {
     const int e = lb.list[s];
+    const int cs = m.el_case64[e >> 6];
+    CaseCtl* cc = ctl.cases + cs;
     double* out = lb.w + (size_t)s * 4 * NP;
-    const int st = weno_wave<T, P, NQ, NE>(m, U, smat, ctl->hcr, e, sh, out);
-    if (st && threadIdx.x == 0) atomicMin(&ctl->error, (0xFFFFFFull << 8) | (unsigned long long)st);
+    const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, e, sh, out);
+    if (st && threadIdx.x == 0) atomicMin(&cc->error, (0xFFFFFFull << 8) | (unsigned long long)st);
     __syncthreads();
     if (fuse_dt && !st && threadIdx.x == 0) {
-      const double d = element_dt<NP>([&](int v, int k) { return out[v * NP + k]; }, ctl->hcr, ctl->cfl,
+      const double d = element_dt<NP>([&](int v, int k) { return out[v * NP + k]; }, m.hcr, cc->cfl,
                                       m.el_geo[G_HMIN * m.nes + e]);
       if (d < 1.0e10)
-        atomicMin(&ctl->dtmin_bits[(ctl->step + 1) & 1][(e % DT_LANES) * DT_STRIDE],
+        atomicMin(dt_slot(ctl, cs, cc->step + 1) + (e % DT_LANES) * DT_STRIDE,
                   (unsigned long long)__double_as_longlong(d));
     }
     __syncthreads();
@@ -1083,10 +1120,8 @@ __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U,
 }
 
 template <class T, int NP, int STAGE>
-__global__ __launch_bounds__(256) void k_apply(LimiterBufs lb, StageBufs<T> b, int nes, Ctl* __restrict__ ctl, int slot,
-                                               int force) {
-  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
-  const int n = ctl->count[slot];
+__global__ __launch_bounds__(256) void k_apply(LimiterBufs lb, StageBufs<T> b, int nes, Ctl ctl, int slot) {
+  const int n = ctl.count[slot];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * 4 * NP; t += gridDim.x * blockDim.x) {
     const int s = t / (4 * NP), idx = t % (4 * NP);
     store_limited<T, NP, STAGE>(b, nes, lb.list[s], idx, lb.w[(size_t)s * 4 * NP + idx]);
@@ -1099,14 +1134,17 @@ template <class T, int P, int NQ, int NE, int STAGE>
 __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __restrict__ elems, int n, StageBufs<T> b,
                                                     const T* __restrict__ U, const double* __restrict__ lt,
                                                     const double* __restrict__ smat, LimiterBufs lb,
-                                                    Ctl* __restrict__ ctl, int slot, int force) {
+                                                    Ctl ctl, int slot, int force) {
   __shared__ WenoShared<P, NQ, NE> sh;
   __shared__ double outv[4 * np_of(P)];
-  if (!force && (!ctl->active || ctl->error != ~0ull)) return;
   constexpr int NP = np_of(P);
   const int s = blockIdx.x;
   if (s >= n) return;
   const int e = elems[s];
+  const int cs = m.el_case64[e >> 6];
+  if (cs < 0) return;
+  CaseCtl* cc = ctl.cases + cs;
+  if (!force && !case_runs(*cc)) return;
   if (threadIdx.x == 0) {
     const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e);
     sh.flag = t ? 1 : 0;
@@ -1114,42 +1152,48 @@ __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __rest
   }
   __syncthreads();
   if (!sh.flag) return;
-  const int st = weno_wave<T, P, NQ, NE>(m, U, smat, ctl->hcr, e, sh, outv);
+  const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, e, sh, outv);
   __syncthreads();
   if (st) {
-    if (threadIdx.x == 0) atomicMin(&ctl->error, ((unsigned long long)(unsigned)e << 8) | (unsigned long long)st);
+    if (threadIdx.x == 0)
+      atomicMin(&cc->error, ((unsigned long long)(unsigned)(e - cc->e_off) << 8) | (unsigned long long)st);
     return;
   }
   for (int idx = threadIdx.x; idx < 4 * NP; idx += 64) store_limited<T, NP, STAGE>(b, m.nes, e, idx, outv[idx]);
-  if (threadIdx.x == 0) atomicAdd(&ctl->count[slot], 1);
+  if (threadIdx.x == 0) {
+    atomicAdd(&ctl.count[slot], 1);
+    atomicAdd(&cc->count[slot], 1);
+  }
 }
 
 // ---------------------------------------------------------------------------------------
 // layout conversion and checks
 // ---------------------------------------------------------------------------------------
 // host layout [ne][4][NP] (double) -> device [4 NP][nes]
+// host layout [n][4][NP] (double) -> device [4 NP][nes] at elements e0 .. e0 + n - 1
 template <class T>
-__global__ void k_to_soa(const double* __restrict__ in, T* __restrict__ out, double* __restrict__ out64, int ne, int nes,
-                         int ncomp) {
+__global__ void k_to_soa(const double* __restrict__ in, T* __restrict__ out, double* __restrict__ out64, int e0, int n,
+                         int nes, int ncomp) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
-  if (e >= ne) return;
+  if (e >= n) return;
   const double v = in[(size_t)e * ncomp + c];
-  if (out) out[(size_t)c * nes + e] = (T)v;
-  if (out64) out64[(size_t)c * nes + e] = v;
+  if (out) out[(size_t)c * nes + e0 + e] = (T)v;
+  if (out64) out64[(size_t)c * nes + e0 + e] = v;
 }
-__global__ void k_from_soa(const double* __restrict__ in, double* __restrict__ out, int ne, int nes, int ncomp) {
+__global__ void k_from_soa(const double* __restrict__ in, double* __restrict__ out, int e0, int n, int nes, int ncomp) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = blockIdx.y;
-  if (e >= ne) return;
-  out[(size_t)e * ncomp + c] = in[(size_t)c * nes + e];
+  if (e >= n) return;
+  out[(size_t)e * ncomp + c] = in[(size_t)c * nes + e0 + e];
 }
 __global__ void k_copy_f64(const double* __restrict__ in, double* __restrict__ out, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in[i];
 }
-__global__ void k_check_finite(const double* __restrict__ u, int ne, int nes, int ncomp, int* flag) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
+__global__ void k_check_finite(const double* __restrict__ u, int e0, int n, int nes, int ncomp, int* flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int e = e0 + i;
   for (int c = 0; c < ncomp; ++c)
     if (!isfinite(u[(size_t)c * nes + e])) {
       atomicOr(flag, 1);
@@ -1157,20 +1201,27 @@ __global__ void k_check_finite(const double* __restrict__ u, int ne, int nes, in
     }
 }
 
-// wall pressure force (Fx, Fy) from the mode coefficients: sum over wall edges and edge points of
-// p * n * w * |e| / 2, with the normal oriented out of the fluid element
+// wall pressure force (Fx, Fy) per case from the mode coefficients: sum over wall edges and edge
+// points of p * n * w * |e| / 2, with the normal oriented out of the fluid element; force[2 case + i].
+// Edges ed0 .. ed0 + n - 1 with ed0 a multiple of 64 (one wavefront = one case).
 template <class T, int P, int NQ, int NE>
-__global__ void k_wall_force(DevMesh m, const double* __restrict__ U, const double* __restrict__ lt, double hcr,
-                             double* __restrict__ force) {
+__global__ void k_wall_force(DevMesh m, const double* __restrict__ U, const double* __restrict__ lt, Ctl ctl, int ed0,
+                             int n, double* __restrict__ force) {
   using LO = LimOff<P, NQ, NE>;
   constexpr int NP = LO::NP;
-  const int ed = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ed = ed0 + i;
+  const int w0 = ed0 + (int)(blockIdx.x * blockDim.x) + (int)(threadIdx.x & ~63u);  // first edge of the wavefront
+  if (w0 >= m.ed_stride) return;
+  const int cs = m.ed_case64[w0 >> 6];
+  if (cs < 0) return;
+  const double hcr = m.hcr;
   double fx = 0.0, fy = 0.0;
-  if (ed < m.nedge) {
+  if (i < n && ed < ctl.cases[cs].ed_end) {
     const int info = m.ed_info[ed];
     if (((info >> 4) & 3) == BC_WALL) {
       const int L = m.ed_elem[ed], lic = info & 3;
-      const double nx = m.ed_geo[ed], ny = m.ed_geo[m.nedge + ed], hl = m.ed_geo[2 * m.nedge + ed];
+      const double nx = m.ed_geo[ed], ny = m.ed_geo[m.ed_stride + ed], hl = m.ed_geo[2 * m.ed_stride + ed];
       for (int q = 0; q < NE; ++q) {
         double s[4] = {0, 0, 0, 0};
         for (int v = 0; v < 4; ++v)
@@ -1186,8 +1237,8 @@ __global__ void k_wall_force(DevMesh m, const double* __restrict__ U, const doub
     fy += __shfl_xor(fy, o, 64);
   }
   if ((threadIdx.x & 63) == 0 && (fx != 0.0 || fy != 0.0)) {
-    atomicAdd(&force[0], fx);
-    atomicAdd(&force[1], fy);
+    atomicAdd(&force[2 * cs], fx);
+    atomicAdd(&force[2 * cs + 1], fy);
   }
 }
 
@@ -1195,7 +1246,9 @@ inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs
 
 }  // namespace
 
-void launch_step_begin(Ctl* ctl, hipStream_t st) { k_step_begin<<<1, 1, 0, st>>>(ctl); }
+void launch_step_begin(Ctl ctl, int ncases, hipStream_t st) {
+  if (ncases > 0) k_step_begin<<<blocks_for(ncases, 64), 64, 0, st>>>(ctl, ncases);
+}
 
 // ---------------------------------------------------------------------------------------
 // Launch sequences for one (precision, order, rule) combination
@@ -1210,6 +1263,7 @@ class SolverImpl final : public Solver {
   int table_len() const override { return O::N; }
   int limiter_table_len() const override { return LO::N; }
   bool fp32() const override { return sizeof(T) == 4; }
+  bool has_mfma() const override { return sizeof(T) == 4 && NP <= 16 && NQ <= 16 && 3 * NE <= 16; }
 
   StageBufs<T> bufs(const DeviceState& s) const {
     StageBufs<T> b;
@@ -1233,20 +1287,25 @@ class SolverImpl final : public Solver {
   }
 
   void surface(const DeviceState& s, const T* in, hipStream_t st, int force) const {
-    k_surface<T, NP, NQ, NE><<<blocks_for(s.mesh.nedge, 256), 256, 0, st>>>(s.mesh, in, (T*)s.flux, (const T*)s.tab,
-                                                                           s.ctl, force);
+    if (s.mesh.nedge <= 0) return;
+    const int bs = s.tune.surface_block;
+    k_surface<T, NP, NQ, NE><<<blocks_for(s.mesh.nedge, bs), bs, 0, st>>>(s.mesh, in, (T*)s.flux, (const T*)s.tab,
+                                                                         s.ctl, force, s.tune.remap);
   }
   template <int OP>
   void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force) const {
+    if (s.mesh.ne <= 0) return;
     if constexpr (sizeof(T) == 4 && NP <= 16 && NQ <= 16 && 3 * NE <= 16) {
       if (s.use_mfma) {
-        k_element_mfma<NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, 64), 256, 0, st>>>(s.mesh, b, (const float*)s.flux,
-                                                                                 (const float*)s.tab, s.ctl, force);
+        const int w = s.tune.element_waves;
+        k_element_mfma<NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, 16 * w), 64 * w, 0, st>>>(
+            s.mesh, b, (const float*)s.flux, (const float*)s.tab, s.ctl, force, s.tune.remap);
         return;
       }
     }
-    k_element<T, NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, 128), 128, 0, st>>>(s.mesh, b, (const T*)s.flux,
-                                                                            (const T*)s.tab, s.ctl, force);
+    const int bs = s.tune.element_block;
+    k_element<T, NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, bs), bs, 0, st>>>(s.mesh, b, (const T*)s.flux,
+                                                                          (const T*)s.tab, s.ctl, force, s.tune.remap);
   }
 
   void residual(const DeviceState& s, hipStream_t st) const override {
@@ -1256,29 +1315,10 @@ class SolverImpl final : public Solver {
     element<OP_RESID>(s, b, st, 1);
   }
 
-  template <int STAGE>
-  void limiter_stage(const DeviceState& s, hipStream_t st, int force, int slot) const {
-    StageBufs<T> b = bufs(s);
-    const T* U = limiter_input(s, STAGE);
-    LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags};
-    if (s.limiter == DGM_LIMITER_JACOBI) {
-      k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force, 0,
-                                                                         s.remap_detect);
-      k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, force, 0);
-      k_apply<T, NP, STAGE><<<s.apply_grid, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot, force);
-    } else if (s.limiter == DGM_LIMITER_SEQUENTIAL) {
-      for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
-        const int n = s.level_offsets[l + 1] - s.level_offsets[l];
-        k_limit_level<T, P, NQ, NE, STAGE><<<n, 64, 0, st>>>(s.mesh, s.levels + s.level_offsets[l], n, b, U, s.ltab,
-                                                             s.smat, lb, s.ctl, slot, force);
-      }
-    }
-  }
-
   void limiter(const DeviceState& s, hipStream_t st) const override {
     // stand-alone SpatialDisc::apply_limiter on the resident u^n
     k_reset_count<<<1, 1, 0, st>>>(s.ctl, 3);
-    limiter_stage<2>(s, st, 1, 3);
+    for (int part = PART_DETECT; part < N_PARTS; ++part) limiter_part<2>(s, st, 1, 3, part);
   }
 
   void stage(const DeviceState& s, int stage_idx, hipStream_t st) const override {
@@ -1310,18 +1350,22 @@ class SolverImpl final : public Solver {
 
   template <int STAGE>
   void limiter_part(const DeviceState& s, hipStream_t st, int force, int slot, int part) const {
+    if (s.mesh.ne <= 0) return;
     StageBufs<T> b = bufs(s);
     const T* U = limiter_input(s, STAGE);
     LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags};
     if (s.limiter == DGM_LIMITER_JACOBI) {
-      const int fuse = STAGE == 2 ? 1 : 0;  // the last stage also reduces the next step's dt
-      if (part == PART_DETECT)
-        k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force,
-                                                                         fuse, s.remap_detect);
-      else if (part == PART_REBUILD)
-        k_weno<T, P, NQ, NE><<<s.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, force, fuse);
-      else if (part == PART_APPLY)
-        k_apply<T, NP, STAGE><<<s.apply_grid, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot, force);
+      // the last stage of a step also reduces the next step's time step (not the stand-alone call)
+      const int fuse = (STAGE == 2 && slot == 2) ? 1 : 0;
+      if (part == PART_DETECT) {
+        const int bs = s.tune.detect_block;
+        k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, bs), bs, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force,
+                                                                       fuse, s.tune.remap);
+      } else if (part == PART_REBUILD) {
+        k_weno<T, P, NQ, NE><<<s.tune.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, fuse);
+      } else if (part == PART_APPLY) {
+        k_apply<T, NP, STAGE><<<1024, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot);
+      }
     } else if (s.limiter == DGM_LIMITER_SEQUENTIAL && part == PART_DETECT) {
       for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
         const int n = s.level_offsets[l + 1] - s.level_offsets[l];
@@ -1332,31 +1376,36 @@ class SolverImpl final : public Solver {
   }
 
   int smoothness_len() const override { return NP * (NP + 1) / 2; }
-  void build_smoothness(const DeviceState& s, hipStream_t st) const override {
-    if (s.mesh.ne > 0) k_smoothness<P, NQ, NE><<<s.mesh.ne, 64, 0, st>>>(s.mesh, s.ltab, s.smat);
+  void build_smoothness(const DeviceState& s, int e0, int n, hipStream_t st) const override {
+    if (n > 0) k_smoothness<P, NQ, NE><<<n, 64, 0, st>>>(s.mesh, s.ltab, s.smat, e0);
   }
 
-  void time_step(const DeviceState& s, hipStream_t st) const override {
+  void time_step(const DeviceState& s, hipStream_t st, int force) const override {
+    if (s.mesh.ne <= 0) return;
     const T* U = sizeof(T) == 8 ? (const T*)s.un : (const T*)s.sn;
-    k_dt<T, NP><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ctl);
+    k_dt<T, NP><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ctl, force);
   }
 
-  void upload(const DeviceState& s, const double* host_layout_dev, hipStream_t st) const override {
-    dim3 g(blocks_for(s.mesh.ne, 256), 4 * NP);
+  void upload(const DeviceState& s, const double* host_layout_dev, int e0, int n, hipStream_t st) const override {
+    if (n <= 0) return;
+    dim3 g(blocks_for(n, 256), 4 * NP);
     if (sizeof(T) == 8)
-      k_to_soa<double><<<g, 256, 0, st>>>(host_layout_dev, nullptr, s.un, s.mesh.ne, s.mesh.nes, 4 * NP);
+      k_to_soa<double><<<g, 256, 0, st>>>(host_layout_dev, nullptr, s.un, e0, n, s.mesh.nes, 4 * NP);
     else
-      k_to_soa<float><<<g, 256, 0, st>>>(host_layout_dev, (float*)s.sn, s.un, s.mesh.ne, s.mesh.nes, 4 * NP);
+      k_to_soa<float><<<g, 256, 0, st>>>(host_layout_dev, (float*)s.sn, s.un, e0, n, s.mesh.nes, 4 * NP);
   }
-  void download(const DeviceState& s, const double* soa, double* host_layout_dev, hipStream_t st) const override {
-    dim3 g(blocks_for(s.mesh.ne, 256), 4 * NP);
-    k_from_soa<<<g, 256, 0, st>>>(soa, host_layout_dev, s.mesh.ne, s.mesh.nes, 4 * NP);
+  void download(const DeviceState& s, const double* soa, double* host_layout_dev, int e0, int n,
+                hipStream_t st) const override {
+    if (n <= 0) return;
+    dim3 g(blocks_for(n, 256), 4 * NP);
+    k_from_soa<<<g, 256, 0, st>>>(soa, host_layout_dev, e0, n, s.mesh.nes, 4 * NP);
   }
-  void check_finite(const DeviceState& s, int* flag, hipStream_t st) const override {
-    k_check_finite<<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.un, s.mesh.ne, s.mesh.nes, 4 * NP, flag);
+  void check_finite(const DeviceState& s, int* flag, int e0, int n, hipStream_t st) const override {
+    if (n > 0) k_check_finite<<<blocks_for(n, 256), 256, 0, st>>>(s.un, e0, n, s.mesh.nes, 4 * NP, flag);
   }
-  void wall_force(const DeviceState& s, double* force, hipStream_t st) const override {
-    k_wall_force<T, P, NQ, NE><<<blocks_for(s.mesh.nedge, 256), 256, 0, st>>>(s.mesh, s.un, s.ltab, s.hcr, force);
+  void wall_force(const DeviceState& s, double* force, int ed0, int n, hipStream_t st) const override {
+    if (n > 0)
+      k_wall_force<T, P, NQ, NE><<<blocks_for(n, 256), 256, 0, st>>>(s.mesh, s.un, s.ltab, s.ctl, ed0, n, force);
   }
 };
 

@@ -1,4 +1,4 @@
-// Launch-sequence interface shared by the C ABI (dgm.cpp) and the kernels (kernels.hip).
+// Launch-sequence interface shared by the C ABI (dgm.cpp, batch.cpp) and the kernels (kernels.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -10,7 +10,8 @@
 
 namespace dgm {
 
-// Device buffers of one context. Sizes: state arrays [4 Np][nes]; flux [n_edges][Ne][4].
+// Device buffers of one arena (one polynomial order; one or many cases). Sizes: state arrays
+// [4 Np][nes]; flux [edge capacity][Ne][4].
 struct DeviceState {
   DevMesh mesh{};
   double* un = nullptr;    // u^n, FP64
@@ -22,18 +23,16 @@ struct DeviceState {
   void* flux = nullptr;
   void* tab = nullptr;     // bulk-precision tables
   double* ltab = nullptr;  // FP64 limiter tables
-  double* smat = nullptr;  // per-element packed smoothness matrices [ne][Np(Np+1)/2]
+  double* smat = nullptr;  // per-element packed smoothness matrices [nes][Np(Np+1)/2]
   int* lim_list = nullptr;
   double* lim_w = nullptr;
   uint8_t* lim_flags = nullptr;
-  Ctl* ctl = nullptr;
+  Ctl ctl{};               // case records (shared by the arenas of a batch) and this arena's list counters
   int limiter = 0;
   int use_mfma = 1;        // FP32-bulk element kernel on the matrix cores (p <= 3)
-  int remap_detect = 1;    // XCD-contiguous block order for the detector
-  unsigned weno_grid = 1, apply_grid = 1;
+  Tuning tune{};
   std::vector<int> level_offsets;
   const int* levels = nullptr;
-  double hcr = 1.4;
 };
 
 class Solver {
@@ -42,30 +41,34 @@ class Solver {
   virtual int table_len() const = 0;
   virtual int limiter_table_len() const = 0;
   virtual bool fp32() const = 0;
+  virtual bool has_mfma() const = 0;
   // residual of the resident u^n into rout (no time control, always runs)
   virtual void residual(const DeviceState& s, hipStream_t st) const = 0;
   // SpatialDisc::apply_limiter on the resident u^n (always runs)
   virtual void limiter(const DeviceState& s, hipStream_t st) const = 0;
-  // RK stage 0, 1, 2 (surface + element + limiter), gated by Ctl::active
+  // RK stage 0, 1, 2 (surface + element + limiter), each case gated by its CaseCtl::active
   virtual void stage(const DeviceState& s, int stage_idx, hipStream_t st) const = 0;
   // the parts of a stage, in order: surface flux, element update, detector (or the whole
   // element-order sweep), WENO rebuild, apply
   enum { PART_SURFACE = 0, PART_ELEMENT = 1, PART_DETECT = 2, PART_REBUILD = 3, PART_APPLY = 4, N_PARTS = 5 };
   virtual void stage_part(const DeviceState& s, int stage_idx, int part, hipStream_t st) const = 0;
-  // per-element smoothness matrices of the WENO indicator (set-up)
+  // per-element smoothness matrices of the WENO indicator (set-up), elements e0 .. e0 + n - 1
   virtual int smoothness_len() const = 0;
-  virtual void build_smoothness(const DeviceState& s, hipStream_t st) const = 0;
-  // time-step reduction into Ctl::dtmin_bits
-  virtual void time_step(const DeviceState& s, hipStream_t st) const = 0;
-  virtual void upload(const DeviceState& s, const double* host_layout_dev, hipStream_t st) const = 0;
-  virtual void download(const DeviceState& s, const double* soa, double* host_layout_dev, hipStream_t st) const = 0;
-  virtual void check_finite(const DeviceState& s, int* flag, hipStream_t st) const = 0;
-  virtual void wall_force(const DeviceState& s, double* force, hipStream_t st) const = 0;
+  virtual void build_smoothness(const DeviceState& s, int e0, int n, hipStream_t st) const = 0;
+  // time-step reduction into the dt slot (step + pending) of every live case (force: also inactive ones)
+  virtual void time_step(const DeviceState& s, hipStream_t st, int force) const = 0;
+  // host layout [n][4][Np] (device copy) <-> state arrays, elements e0 .. e0 + n - 1
+  virtual void upload(const DeviceState& s, const double* host_layout_dev, int e0, int n, hipStream_t st) const = 0;
+  virtual void download(const DeviceState& s, const double* soa, double* host_layout_dev, int e0, int n,
+                        hipStream_t st) const = 0;
+  virtual void check_finite(const DeviceState& s, int* flag, int e0, int n, hipStream_t st) const = 0;
+  // wall force per case into force[2 case + i], edges ed0 .. ed0 + n - 1 (ed0 a multiple of 64)
+  virtual void wall_force(const DeviceState& s, double* force, int ed0, int n, hipStream_t st) const = 0;
 };
 
 std::unique_ptr<Solver> make_solver(int precision, int p, int nq, int ne);
 
-// kernel launches with single-thread control
-void launch_step_begin(Ctl* ctl, hipStream_t st);
+// per-case step control: one thread per case slot
+void launch_step_begin(Ctl ctl, int ncases, hipStream_t st);
 
 }  // namespace dgm

@@ -40,7 +40,8 @@ typedef enum {
   DGM_BAD_MESH = 5,      /* mesh the reference would panic on */
   DGM_HIP_ERROR = 6,
   DGM_OUT_OF_MEMORY = 7,
-  DGM_BAD_ARGUMENT = 8
+  DGM_BAD_ARGUMENT = 8,
+  DGM_NO_SPACE = 9       /* batch: the case does not fit the free capacity now (admission refused) */
 } dgm_status;
 
 enum { DGM_FP64 = 0, DGM_FP32_BULK = 1 };
