@@ -7,3 +7,4 @@ The drop-in surface mirrors the Rust crate Decem5150/DGM_in_rust: :class:`Spatia
 from .meshgen import MeshInput, naca0012_ogrid, periodic_square, read_gmsh, write_gmsh  # noqa: F401
 from .solver import Context, Solver, SpatialDisc, TemperalDisc  # noqa: F401
 from ._lib import DgmError  # noqa: F401
+from .batch import Batch  # noqa: F401

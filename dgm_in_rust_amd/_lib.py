@@ -13,7 +13,8 @@ _f64p = C.POINTER(C.c_double)
 _u8p = C.POINTER(C.c_uint8)
 
 STATUS = {0: "OK", 1: "NEG_DENSITY", 2: "NEG_PRESSURE", 3: "NONFINITE", 4: "BAD_CONFIG", 5: "BAD_MESH",
-          6: "HIP_ERROR", 7: "OUT_OF_MEMORY", 8: "BAD_ARGUMENT"}
+          6: "HIP_ERROR", 7: "OUT_OF_MEMORY", 8: "BAD_ARGUMENT", 9: "NO_SPACE"}
+NO_SPACE = 9
 
 FP64, FP32_BULK = 0, 1
 LIMITER_OFF, LIMITER_SEQUENTIAL, LIMITER_JACOBI = 0, 1, 2
@@ -23,7 +24,11 @@ EXPORTED = ["dgm_ctx_create", "dgm_ctx_create_host", "dgm_ctx_destroy", "dgm_ctx
             "dgm_compute_residuals", "dgm_apply_limiter", "dgm_compute_time_step", "dgm_rk3", "dgm_time_march",
             "dgm_state_upload", "dgm_state_download", "dgm_device_time_march", "dgm_device_steps_async",
             "dgm_synchronize", "dgm_stream", "dgm_check_state", "dgm_device_clock", "dgm_lift_coefficient",
-            "dgm_profile_steps", "dgm_troubled_counts"]
+            "dgm_profile_steps", "dgm_troubled_counts",
+            "dgm_batch_create", "dgm_batch_destroy", "dgm_batch_last_error", "dgm_batch_footprint", "dgm_batch_add",
+            "dgm_batch_remove", "dgm_batch_steps_async", "dgm_batch_synchronize", "dgm_batch_case_info",
+            "dgm_batch_case_state", "dgm_batch_lift", "dgm_batch_get_tuning", "dgm_batch_set_tuning",
+            "dgm_batch_autotune"]
 
 
 class Mesh(C.Structure):
@@ -60,6 +65,32 @@ class KernelTimes(C.Structure):
     _fields_ = [("surface_ms", C.c_double), ("element_ms", C.c_double), ("detect_ms", C.c_double),
                 ("rebuild_ms", C.c_double), ("apply_ms", C.c_double), ("time_step_ms", C.c_double),
                 ("control_ms", C.c_double), ("step_ms", C.c_double), ("steps", C.c_int32)]
+
+
+class BatchParams(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("limiter", C.c_int32), ("hcr", C.c_double), ("max_cases", C.c_int32),
+                ("pad", C.c_int32), ("capacity", C.c_int64 * 4)]
+
+
+class CaseParams(C.Structure):
+    _fields_ = [("order", C.c_int32), ("final_step", C.c_int32), ("final_time", C.c_double), ("cfl", C.c_double),
+                ("fs_mach", C.c_double), ("fs_aoa_deg", C.c_double), ("fs_rho", C.c_double), ("fs_p", C.c_double),
+                ("fs_hcr", C.c_double)]
+
+
+class CaseInfo(C.Structure):
+    _fields_ = [("live", C.c_int32), ("active", C.c_int32), ("status", C.c_int32), ("steps", C.c_int32),
+                ("time", C.c_double), ("last_dt", C.c_double), ("error_edge", C.c_int64),
+                ("troubled_last_stage", C.c_int32), ("order", C.c_int32), ("n_elements", C.c_int64),
+                ("footprint_bytes", C.c_int64)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("surface_block", C.c_int32), ("element_waves", C.c_int32), ("element_block", C.c_int32),
+                ("detect_block", C.c_int32), ("remap", C.c_int32), ("weno_grid", C.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class DgmError(RuntimeError):
@@ -110,8 +141,30 @@ def lib():
     L.dgm_lift_coefficient.argtypes = [vp, C.c_double, _f64p]
     L.dgm_troubled_counts.argtypes = [vp, _i32p]
     L.dgm_profile_steps.argtypes = [vp, C.c_int32, C.POINTER(KernelTimes)]
+    L.dgm_batch_create.argtypes = [C.POINTER(BatchParams), C.POINTER(vp)]
+    L.dgm_batch_destroy.argtypes = [vp]
+    L.dgm_batch_destroy.restype = None
+    L.dgm_batch_last_error.argtypes = [vp, C.POINTER(Error)]
+    L.dgm_batch_footprint.argtypes = [vp, C.c_int32, C.c_int64, C.c_int64, C.c_int64, _i64p]
+    L.dgm_batch_add.argtypes = [vp, C.POINTER(Mesh), C.POINTER(CaseParams), _f64p, _i32p]
+    L.dgm_batch_remove.argtypes = [vp, C.c_int32]
+    L.dgm_batch_steps_async.argtypes = [vp, C.c_int32]
+    L.dgm_batch_synchronize.argtypes = [vp]
+    L.dgm_batch_case_info.argtypes = [vp, C.c_int32, C.POINTER(CaseInfo)]
+    L.dgm_batch_case_state.argtypes = [vp, C.c_int32, _f64p]
+    L.dgm_batch_lift.argtypes = [vp, C.c_double, C.c_int32, _f64p]
+    L.dgm_batch_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(Tuning)]
+    L.dgm_batch_set_tuning.argtypes = [vp, C.c_int32, C.POINTER(Tuning)]
+    L.dgm_batch_autotune.argtypes = [vp, C.c_int32, C.POINTER(Tuning)]
     _lib = L
     return L
+
+
+def mesh_struct(m):
+    """dgm_mesh view of a contiguous MeshInput (the arrays must outlive the struct)."""
+    return Mesh(m.vxy.shape[0], ptr(m.vxy), len(m.eb_type), ptr(m.eb_type, _i32p), ptr(m.eb_count, _i64p),
+                ptr(m.eb_nodes, _i64p), len(m.el_count), ptr(m.el_count, _i64p), ptr(m.el_nodes, _i64p),
+                ptr(m.elem_xy) if m.elem_xy is not None else None)
 
 
 def ptr(a, t=_f64p):

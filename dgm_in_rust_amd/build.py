@@ -20,7 +20,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-Wno-u
           "-I" + os.path.join(HERE, "..", "include")]
 HIPFLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-munsafe-fp-atomics"]
-SOURCES = [("kernels.hip", True), ("dgm.cpp", True), ("arena.cpp", True), ("tables.cpp", False), ("mesh.cpp", False)]
+SOURCES = [("kernels.hip", True), ("dgm.cpp", True), ("arena.cpp", True), ("batch.cpp", True), ("tables.cpp", False),
+           ("mesh.cpp", False)]
 
 
 def _compile(src: str, hip: bool) -> str:

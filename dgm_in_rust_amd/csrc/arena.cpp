@@ -93,6 +93,29 @@ CaseCtl fresh_case(double cfl) {
   return h;
 }
 
+void initial_state(const Tables& T, const HostMesh& M, double density, double aoa_deg, double mach, double pressure,
+                   double hcr, double* U) {
+  const double a = std::sqrt(hcr * pressure / density);
+  const double rad = aoa_deg * (M_PI / 180.0);
+  const double u = mach * std::cos(rad) * a, v = mach * std::sin(rad) * a;
+  const double q[4] = {density, density * u, density * v, pressure / (hcr - 1.0) + 0.5 * density * (u * u + v * v)};
+  for (int64_t e = 0; e < M.ne; ++e) {
+    const double hj = 0.5 * M.el_J[e];
+    for (int var = 0; var < 4; ++var)
+      for (int k = 0; k < T.np; ++k) {
+        double s = 0.0;
+        for (int i = 0; i < T.nq; ++i) s += hj * T.cell_w[i] * q[var] * T.phi_cell[i * T.np + k];
+        U[(e * 4 + var) * T.np + k] = s / (T.mass[k] * hj);
+      }
+  }
+}
+
+double default_cfl(int order, double cfl) {
+  if (cfl > 0.0) return cfl;
+  const double p = (double)order;
+  return 1.0 / ((2.0 * p + 1.0) * (1.0 + 4.0 / ((p + 2.0) * (p + 2.0))));
+}
+
 // ---- Arena ----
 
 Footprint Arena::footprint(int order, int nq, int ne_pts, int precision, int limiter, bool residual_buffer) {

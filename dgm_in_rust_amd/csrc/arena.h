@@ -102,4 +102,11 @@ std::vector<double> farfield_states(const HostMesh& M, double mach, double aoa_d
 // Empty control record of a case.
 CaseCtl fresh_case(double cfl);
 
+// Solver::set_initial_solution (src/solver.rs:45-79): L2 projection of a uniform state, host layout.
+void initial_state(const Tables& T, const HostMesh& M, double density, double aoa_deg, double mach, double pressure,
+                   double hcr, double* U);
+
+// CFL of the reference for order p (src/initialization.rs:128-132), or the given value when > 0.
+double default_cfl(int order, double cfl);
+
 }  // namespace dgm

@@ -1,0 +1,542 @@
+// Batches of cases (include/dgm.h, dgm_batch_*): one arena per order, cases added and removed between
+// steps, one captured graph per step covering every running case, per-case control records, lift,
+// and an autotuner of the launch shapes of each order's kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "arena.h"
+#include "dgm_internal.h"
+#include "solver.h"
+
+using namespace dgm;
+
+namespace {
+
+struct BatchCase {
+  bool live = false;
+  int order = 0;
+  Placement pl;
+  dgm_case_params cp{};
+  double cfl = 0.0;
+  int64_t footprint = 0;
+};
+
+}  // namespace
+
+struct dgm_batch {
+  dgm_batch_params prm{};
+  hipStream_t stream = nullptr;
+  CaseCtl* d_cases = nullptr;             // [max_cases]
+  unsigned long long* d_dtmin = nullptr;  // [max_cases][DT_CASE]
+  double* d_force = nullptr;              // [2 max_cases]
+  CaseCtl* h_cases = nullptr;             // pinned mirror
+  unsigned long long* h_dt = nullptr;     // pinned, one case's dt slots (all empty)
+  std::vector<void*> allocs;
+  std::unique_ptr<Arena> arena[4];
+  std::vector<BatchCase> cases;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int64_t captured[4][2] = {};            // arena extents the graph was captured with
+  dgm_error err{};
+};
+
+namespace {
+
+int fail(dgm_batch* b, int status, const std::string& msg, int64_t edge = -1) {
+  if (b) {
+    b->err.status = status;
+    b->err.edge = edge;
+    b->err.element = -1;
+    std::snprintf(b->err.message, sizeof(b->err.message), "%s", msg.c_str());
+  }
+  return status;
+}
+
+#define BCHK(b, call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) return fail((b), DGM_HIP_ERROR, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class X>
+int balloc(dgm_batch* b, X** p, size_t n) {
+  *p = nullptr;
+  void* v = nullptr;
+  const hipError_t e = hipMalloc(&v, std::max<size_t>(1, n) * sizeof(X));
+  if (e != hipSuccess) return fail(b, DGM_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+  b->allocs.push_back(v);
+  *p = (X*)v;
+  return DGM_OK;
+}
+
+Ctl shared_ctl(dgm_batch* b) { return Ctl{b->d_cases, b->d_dtmin, nullptr}; }
+
+// writes one case record and empties its dt slots (stream-ordered; returns when done)
+int put_case(dgm_batch* b, int id, const CaseCtl& h) {
+  std::memcpy(&b->h_cases[id], &h, sizeof(CaseCtl));
+  BCHK(b, hipMemcpyAsync(b->d_cases + id, &b->h_cases[id], sizeof(CaseCtl), hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipMemcpyAsync(b->d_dtmin + (size_t)id * DT_CASE, b->h_dt, DT_CASE * 8, hipMemcpyHostToDevice, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  return DGM_OK;
+}
+
+bool graph_current(const dgm_batch* b) {
+  if (!b->exec) return false;
+  for (int g = 0; g < 4; ++g) {
+    const int64_t ne = b->arena[g] ? b->arena[g]->extent_elements() : 0;
+    const int64_t ned = b->arena[g] ? b->arena[g]->extent_edges() : 0;
+    if (ne != b->captured[g][0] || ned != b->captured[g][1]) return false;
+  }
+  return true;
+}
+
+void drop_graph(dgm_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void enqueue_step(dgm_batch* b) {
+  if (b->prm.limiter != DGM_LIMITER_JACOBI)
+    for (int g = 0; g < 4; ++g)
+      if (b->arena[g] && b->arena[g]->extent_elements() > 0) b->arena[g]->solver->time_step(b->arena[g]->S, b->stream, 0);
+  launch_step_begin(shared_ctl(b), b->prm.max_cases, b->stream);
+  for (int s = 0; s < 3; ++s)
+    for (int g = 0; g < 4; ++g)
+      if (b->arena[g] && b->arena[g]->extent_elements() > 0) b->arena[g]->solver->stage(b->arena[g]->S, s, b->stream);
+}
+
+int ensure_graph(dgm_batch* b) {
+  if (graph_current(b)) return DGM_OK;
+  drop_graph(b);
+  BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(b);
+  BCHK(b, hipStreamEndCapture(b->stream, &b->graph));
+  BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  for (int g = 0; g < 4; ++g) {
+    b->captured[g][0] = b->arena[g] ? b->arena[g]->extent_elements() : 0;
+    b->captured[g][1] = b->arena[g] ? b->arena[g]->extent_edges() : 0;
+  }
+  return DGM_OK;
+}
+
+int case_status_code(const CaseCtl& h, int64_t* edge) {
+  *edge = -1;
+  if (h.error == ~0ull) return DGM_OK;
+  const int code = (int)(h.error & 0xff);
+  const int64_t e = (int64_t)(h.error >> 8);
+  *edge = e == 0xFFFFFF ? -1 : e;
+  if (code == 1) return DGM_NEG_DENSITY;
+  if (code == 2) return DGM_NEG_PRESSURE;
+  if (code == 5) return DGM_BAD_MESH;
+  return DGM_HIP_ERROR;
+}
+
+Tuning to_tuning(const dgm_tuning& t) {
+  Tuning u;
+  u.surface_block = t.surface_block;
+  u.element_waves = t.element_waves;
+  u.element_block = t.element_block;
+  u.detect_block = t.detect_block;
+  u.remap = t.remap;
+  u.weno_grid = t.weno_grid;
+  return u;
+}
+dgm_tuning from_tuning(const Tuning& u) {
+  dgm_tuning t;
+  t.surface_block = u.surface_block;
+  t.element_waves = u.element_waves;
+  t.element_block = u.element_block;
+  t.detect_block = u.detect_block;
+  t.remap = u.remap;
+  t.weno_grid = u.weno_grid;
+  return t;
+}
+bool valid_block(int v) { return v == 64 || v == 128 || v == 256; }
+bool valid_tuning(const dgm_tuning& t) {
+  return valid_block(t.surface_block) && (t.element_waves == 1 || t.element_waves == 2 || t.element_waves == 4) &&
+         valid_block(t.element_block) && valid_block(t.detect_block) && (t.remap == 0 || t.remap == 1) &&
+         t.weno_grid >= 64 && t.weno_grid <= 65536;
+}
+
+// Mean milliseconds of the five stage-0 parts of one arena under launch shapes `t`. Stage-0 passes
+// read u^n (and the case's current dt) and write only stage buffers, the flux, the troubled list and
+// the stage-0 counters, so they leave every case's state and clock as they were.
+int time_stage0(dgm_batch* b, Arena& A, const Tuning& t, int reps, double ms[Solver::N_PARTS]) {
+  A.S.tune = t;
+  std::vector<hipEvent_t> ev(Solver::N_PARTS + 1);
+  for (auto& e : ev) BCHK(b, hipEventCreate(&e));
+  for (int p = 0; p < Solver::N_PARTS; ++p) ms[p] = 0.0;
+  for (int r = 0; r < reps + 1; ++r) {  // the first pass warms up
+    BCHK(b, hipEventRecord(ev[0], b->stream));
+    for (int p = 0; p < Solver::N_PARTS; ++p) {
+      A.solver->stage_part(A.S, 0, p, b->stream);
+      BCHK(b, hipEventRecord(ev[p + 1], b->stream));
+    }
+    BCHK(b, hipEventSynchronize(ev[Solver::N_PARTS]));
+    BCHK(b, hipGetLastError());
+    if (r == 0) continue;
+    for (int p = 0; p < Solver::N_PARTS; ++p) {
+      float x = 0.f;
+      BCHK(b, hipEventElapsedTime(&x, ev[p], ev[p + 1]));
+      ms[p] += x / reps;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return DGM_OK;
+}
+
+double total(const double ms[Solver::N_PARTS]) {
+  double s = 0.0;
+  for (int p = 0; p < Solver::N_PARTS; ++p) s += ms[p];
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgm_batch_create(const dgm_batch_params* params, dgm_batch** out) {
+  if (!params || !out) return DGM_BAD_ARGUMENT;
+  *out = nullptr;
+  const dgm_batch_params& P = *params;
+  if (P.precision != DGM_FP64 && P.precision != DGM_FP32_BULK) return DGM_BAD_CONFIG;
+  if (P.limiter != DGM_LIMITER_OFF && P.limiter != DGM_LIMITER_JACOBI) return DGM_BAD_CONFIG;
+  if (!(P.hcr > 1.0) || P.max_cases < 1 || P.max_cases > (1 << 20)) return DGM_BAD_CONFIG;
+  int64_t total_cap = 0;
+  for (int g = 0; g < 4; ++g) {
+    if (P.capacity[g] < 0) return DGM_BAD_CONFIG;
+    total_cap += P.capacity[g];
+  }
+  if (total_cap <= 0) return DGM_BAD_CONFIG;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return DGM_HIP_ERROR;
+  std::unique_ptr<dgm_batch> b(new dgm_batch());
+  b->prm = P;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return DGM_HIP_ERROR;
+  const int nc = P.max_cases;
+  int s = DGM_OK;
+  auto cleanup = [&](int st) {
+    dgm_batch_destroy(b.release());
+    return st;
+  };
+  if ((s = balloc(b.get(), &b->d_cases, nc))) return cleanup(s);
+  if ((s = balloc(b.get(), &b->d_dtmin, (size_t)nc * DT_CASE))) return cleanup(s);
+  if ((s = balloc(b.get(), &b->d_force, 2 * (size_t)nc))) return cleanup(s);
+  if (hipHostMalloc((void**)&b->h_cases, sizeof(CaseCtl) * nc, hipHostMallocDefault) != hipSuccess) return cleanup(DGM_OUT_OF_MEMORY);
+  if (hipHostMalloc((void**)&b->h_dt, DT_CASE * 8, hipHostMallocDefault) != hipSuccess) return cleanup(DGM_OUT_OF_MEMORY);
+  std::memset(b->h_cases, 0, sizeof(CaseCtl) * nc);
+  {
+    const double big = 1.0e10;
+    unsigned long long bits;
+    std::memcpy(&bits, &big, 8);
+    for (int i = 0; i < DT_CASE; ++i) b->h_dt[i] = bits;
+    std::vector<unsigned long long> all((size_t)nc * DT_CASE, bits);
+    if (hipMemcpy(b->d_dtmin, all.data(), all.size() * 8, hipMemcpyHostToDevice) != hipSuccess) return cleanup(DGM_HIP_ERROR);
+    if (hipMemcpy(b->d_cases, b->h_cases, sizeof(CaseCtl) * nc, hipMemcpyHostToDevice) != hipSuccess) return cleanup(DGM_HIP_ERROR);
+  }
+  for (int g = 0; g < 4; ++g) {
+    if (P.capacity[g] <= 0) continue;
+    const int order = g + 1;
+    const int64_t cap_e = P.capacity[g];
+    // edges: an O-grid or periodic triangulation has 1.5 edges per triangle plus a boundary row; each
+    // case also rounds its range up to 64
+    const int64_t cap_ed = cap_e * 8 / 5 + 64 * (int64_t)std::min(nc, 4096);
+    b->arena[g].reset(new Arena());
+    std::string e;
+    if ((s = b->arena[g]->init(order, default_cell_rule(order), default_edge_rule(order), P.precision, P.limiter, cap_e,
+                               cap_ed, 8 * (int64_t)nc, false, b->d_cases, b->d_dtmin, b->stream, e))) {
+      std::fprintf(stderr, "dgm_batch_create: %s\n", e.c_str());
+      return cleanup(s);
+    }
+    b->arena[g]->S.mesh.hcr = P.hcr;
+  }
+  b->cases.resize(nc);
+  *out = b.release();
+  return DGM_OK;
+}
+
+void dgm_batch_destroy(dgm_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  for (auto& a : b->arena) a.reset();
+  for (void* p : b->allocs) (void)hipFree(p);
+  if (b->h_cases) (void)hipHostFree(b->h_cases);
+  if (b->h_dt) (void)hipHostFree(b->h_dt);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int dgm_batch_last_error(const dgm_batch* b, dgm_error* e) {
+  if (!b || !e) return DGM_BAD_ARGUMENT;
+  *e = b->err;
+  return DGM_OK;
+}
+
+int dgm_batch_footprint(const dgm_batch* b, int32_t order, int64_t ne, int64_t ned, int64_t npatch, int64_t* bytes) {
+  if (!b || !bytes || order < 1 || order > 4) return DGM_BAD_ARGUMENT;
+  *bytes = Arena::footprint(order, default_cell_rule(order), default_edge_rule(order), b->prm.precision,
+                            b->prm.limiter, false)
+               .of(ne, ned, npatch);
+  return DGM_OK;
+}
+
+int dgm_batch_add(dgm_batch* b, const dgm_mesh* mesh, const dgm_case_params* cp, const double* U0, int32_t* case_id) {
+  if (!b || !mesh || !cp || !case_id) return DGM_BAD_ARGUMENT;
+  *case_id = -1;
+  if (cp->order < 1 || cp->order > 4 || !b->arena[cp->order - 1])
+    return fail(b, DGM_BAD_CONFIG, "the batch has no arena for this order");
+  if (!(cp->fs_rho > 0.0) || !(cp->fs_p > 0.0) || cp->final_step < 0)
+    return fail(b, DGM_BAD_CONFIG, "bad case parameters");
+  int slot = -1;
+  for (int i = 0; i < b->prm.max_cases; ++i)
+    if (!b->cases[i].live) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) return fail(b, DGM_NO_SPACE, "no free case slot");
+  Arena& A = *b->arena[cp->order - 1];
+  HostMesh M;
+  std::string err;
+  if (int s = build_mesh(*mesh, M, err)) return fail(b, s, err);
+  std::vector<double> U;
+  if (!U0) {
+    U.resize((size_t)M.ne * 4 * A.T.np);
+    initial_state(A.T, M, cp->fs_rho, cp->fs_aoa_deg, cp->fs_mach, cp->fs_p, b->prm.hcr, U.data());
+    U0 = U.data();
+  }
+  const std::vector<double> pq = farfield_states(M, cp->fs_mach, cp->fs_aoa_deg, cp->fs_rho, cp->fs_p, cp->fs_hcr,
+                                                 b->prm.hcr);
+  Placement pl;
+  if (int s = A.place(M, pq, slot, pl, err)) return fail(b, s, err.empty() ? "no free range in the arena" : err);
+  A.update_extent();  // the kernels below (and every later graph) must cover the new range
+  if (int s = A.upload_state(U0, pl, err)) return fail(b, s, err);
+  const double cfl = default_cfl(cp->order, cp->cfl);
+  CaseCtl h = fresh_case(cfl);
+  h.e_off = (int)pl.e_off;
+  h.e_end = (int)(pl.e_off + pl.ne);
+  h.ed_off = (int)pl.ed_off;
+  h.ed_end = (int)(pl.ed_off + pl.ned);
+  h.group = cp->order - 1;
+  h.final_step = cp->final_step;
+  h.final_time = cp->final_time;
+  if (int s = put_case(b, slot, h)) return s;
+  // the first step's time step, of this case only (the other cases' slots hold their fused reductions)
+  A.solver->time_step(A.S, b->stream, 0, slot);
+  BCHK(b, hipGetLastError());
+  BCHK(b, hipStreamSynchronize(b->stream));
+  BatchCase& bc = b->cases[slot];
+  bc.live = true;
+  bc.order = cp->order;
+  bc.pl = pl;
+  bc.cp = *cp;
+  bc.cfl = cfl;
+  int64_t fp = 0;
+  dgm_batch_footprint(b, cp->order, pl.ne, pl.ned, pl.npatch, &fp);
+  bc.footprint = fp;
+  *case_id = slot;
+  return DGM_OK;
+}
+
+int dgm_batch_remove(dgm_batch* b, int32_t id) {
+  if (!b || id < 0 || id >= b->prm.max_cases || !b->cases[id].live) return DGM_BAD_ARGUMENT;
+  BatchCase& bc = b->cases[id];
+  CaseCtl h;
+  std::memset(&h, 0, sizeof(h));
+  h.error = ~0ull;
+  if (int s = put_case(b, id, h)) return s;
+  Arena& A = *b->arena[bc.order - 1];
+  std::string err;
+  if (int s = A.unplace(bc.pl, err)) return fail(b, s, err);
+  A.update_extent();
+  bc = BatchCase();
+  return DGM_OK;
+}
+
+int dgm_batch_steps_async(dgm_batch* b, int32_t n) {
+  if (!b || n < 0) return DGM_BAD_ARGUMENT;
+  bool any = false;
+  for (int g = 0; g < 4; ++g) any = any || (b->arena[g] && b->arena[g]->extent_elements() > 0);
+  if (!any || n == 0) return DGM_OK;
+  if (int s = ensure_graph(b)) return s;
+  for (int i = 0; i < n; ++i) BCHK(b, hipGraphLaunch(b->exec, b->stream));
+  return DGM_OK;
+}
+
+int dgm_batch_synchronize(dgm_batch* b) {
+  if (!b) return DGM_BAD_ARGUMENT;
+  BCHK(b, hipStreamSynchronize(b->stream));
+  return DGM_OK;
+}
+
+int dgm_batch_case_info(dgm_batch* b, int32_t n, dgm_case_info* out) {
+  if (!b || !out || n < 0 || n > b->prm.max_cases) return DGM_BAD_ARGUMENT;
+  if (n == 0) return DGM_OK;
+  BCHK(b, hipMemcpyAsync(b->h_cases, b->d_cases, sizeof(CaseCtl) * n, hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i) {
+    const CaseCtl& h = b->h_cases[i];
+    const BatchCase& bc = b->cases[i];
+    dgm_case_info& o = out[i];
+    std::memset(&o, 0, sizeof(o));
+    o.error_edge = -1;
+    if (!bc.live) continue;
+    const double t = h.pending ? h.t + h.dt : h.t;
+    const int32_t steps = h.step + (h.pending ? 1 : 0);
+    o.live = 1;
+    o.status = case_status_code(h, &o.error_edge);
+    o.active = (o.status == DGM_OK && steps < h.final_step && t < h.final_time) ? 1 : 0;
+    o.steps = steps;
+    o.time = t;
+    o.last_dt = h.dt;
+    o.troubled_last_stage = h.pending ? h.count[2] : h.last_count;
+    o.order = bc.order;
+    o.n_elements = bc.pl.ne;
+    o.footprint_bytes = bc.footprint;
+  }
+  return DGM_OK;
+}
+
+int dgm_batch_case_state(dgm_batch* b, int32_t id, double* U) {
+  if (!b || !U || id < 0 || id >= b->prm.max_cases || !b->cases[id].live) return DGM_BAD_ARGUMENT;
+  Arena& A = *b->arena[b->cases[id].order - 1];
+  std::string err;
+  if (int s = A.download_state(A.S.un, U, b->cases[id].pl, err)) return fail(b, s, err);
+  return DGM_OK;
+}
+
+int dgm_batch_lift(dgm_batch* b, double chord, int32_t n, double* cl) {
+  if (!b || !cl || !(chord > 0.0) || n < 0 || n > b->prm.max_cases) return DGM_BAD_ARGUMENT;
+  BCHK(b, hipMemsetAsync(b->d_force, 0, 2 * sizeof(double) * b->prm.max_cases, b->stream));
+  for (int g = 0; g < 4; ++g)
+    if (b->arena[g] && b->arena[g]->extent_edges() > 0)
+      b->arena[g]->solver->wall_force(b->arena[g]->S, b->d_force, 0, (int)b->arena[g]->extent_edges(), b->stream);
+  BCHK(b, hipGetLastError());
+  std::vector<double> f(2 * (size_t)std::max(n, 1));
+  if (n > 0) BCHK(b, hipMemcpyAsync(f.data(), b->d_force, 2 * sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
+  BCHK(b, hipStreamSynchronize(b->stream));
+  for (int i = 0; i < n; ++i) {
+    const BatchCase& bc = b->cases[i];
+    if (!bc.live) {
+      cl[i] = std::nan("");
+      continue;
+    }
+    const dgm_case_params& P = bc.cp;
+    const double a = std::sqrt(P.fs_hcr * P.fs_p / P.fs_rho);
+    const double V = P.fs_mach * a;
+    const double rad = P.fs_aoa_deg * (M_PI / 180.0);
+    cl[i] = (f[2 * i + 1] * std::cos(rad) - f[2 * i] * std::sin(rad)) / (0.5 * P.fs_rho * V * V * chord);
+  }
+  return DGM_OK;
+}
+
+int dgm_batch_get_tuning(const dgm_batch* b, int32_t order, dgm_tuning* t) {
+  if (!b || !t || order < 1 || order > 4 || !b->arena[order - 1]) return DGM_BAD_ARGUMENT;
+  *t = from_tuning(b->arena[order - 1]->S.tune);
+  return DGM_OK;
+}
+
+int dgm_batch_set_tuning(dgm_batch* b, int32_t order, const dgm_tuning* t) {
+  if (!b || !t || order < 1 || order > 4 || !b->arena[order - 1] || !valid_tuning(*t)) return DGM_BAD_ARGUMENT;
+  BCHK(b, hipStreamSynchronize(b->stream));
+  b->arena[order - 1]->S.tune = to_tuning(*t);
+  drop_graph(b);
+  return DGM_OK;
+}
+
+int dgm_batch_autotune(dgm_batch* b, int32_t reps, dgm_tuning chosen[4]) {
+  if (!b || reps < 1) return DGM_BAD_ARGUMENT;
+  BCHK(b, hipStreamSynchronize(b->stream));
+  for (int g = 0; g < 4; ++g) {
+    if (!b->arena[g]) continue;
+    Arena& A = *b->arena[g];
+    if (A.extent_elements() > 0) {
+      Tuning best = A.S.tune;
+      double ms[Solver::N_PARTS];
+      const int blocks[3] = {64, 128, 256};
+      const int waves[3] = {1, 2, 4};
+      const int grids[5] = {256, 512, 1024, 2048, 4096};
+      // one knob at a time, each judged by the part it shapes
+      {
+        double bt = 1e30;
+        Tuning cand = best;
+        for (int v : blocks) {
+          cand.surface_block = v;
+          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
+          if (ms[Solver::PART_SURFACE] < bt) {
+            bt = ms[Solver::PART_SURFACE];
+            best.surface_block = v;
+          }
+        }
+      }
+      {
+        double bt = 1e30;
+        Tuning cand = best;
+        const bool mfma = A.S.use_mfma && A.solver->has_mfma();
+        for (int i = 0; i < 3; ++i) {
+          if (mfma)
+            cand.element_waves = waves[i];
+          else
+            cand.element_block = blocks[i];
+          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
+          if (ms[Solver::PART_ELEMENT] < bt) {
+            bt = ms[Solver::PART_ELEMENT];
+            if (mfma)
+              best.element_waves = waves[i];
+            else
+              best.element_block = blocks[i];
+          }
+        }
+      }
+      if (A.limiter == DGM_LIMITER_JACOBI) {
+        double bt = 1e30;
+        Tuning cand = best;
+        for (int v : blocks) {
+          cand.detect_block = v;
+          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
+          if (ms[Solver::PART_DETECT] < bt) {
+            bt = ms[Solver::PART_DETECT];
+            best.detect_block = v;
+          }
+        }
+        bt = 1e30;
+        cand = best;
+        for (int v : grids) {
+          cand.weno_grid = v;
+          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
+          if (ms[Solver::PART_REBUILD] < bt) {
+            bt = ms[Solver::PART_REBUILD];
+            best.weno_grid = v;
+          }
+        }
+      }
+      {
+        double bt = 1e30;
+        Tuning cand = best;
+        for (int v = 0; v < 2; ++v) {
+          cand.remap = v;
+          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
+          if (total(ms) < bt) {
+            bt = total(ms);
+            best.remap = v;
+          }
+        }
+      }
+      A.S.tune = best;
+    }
+    if (chosen) chosen[g] = from_tuning(A.S.tune);
+  }
+  drop_graph(b);
+  return DGM_OK;
+}
+
+}  // extern "C"

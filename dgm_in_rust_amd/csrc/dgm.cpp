@@ -211,12 +211,7 @@ int create(const dgm_mesh* mesh, const dgm_params* params, dgm_ctx** out, bool d
   std::string err;
   if (int s = build_mesh(*mesh, c->M, err)) return s;
   if (c->M.ne >= (1 << 29) || c->M.nedge >= (1 << 28)) return DGM_BAD_MESH;
-  if (P.cfl > 0.0) {
-    c->cfl = P.cfl;
-  } else {
-    const double p = (double)P.order;  // src/initialization.rs:128-132
-    c->cfl = 1.0 / ((2.0 * p + 1.0) * (1.0 + 4.0 / ((p + 2.0) * (p + 2.0))));
-  }
+  c->cfl = default_cfl(P.order, P.cfl);
   if (!device) {
     *out = c.release();
     return DGM_OK;
@@ -337,21 +332,7 @@ int dgm_get_tables(const dgm_ctx* c, double* phi_cell, double* phi_edge, double*
 // Solver::set_initial_solution, src/solver.rs:45-79
 int dgm_initial_solution(dgm_ctx* c, double density, double aoa_deg, double mach, double pressure, double* U) {
   if (!c || !U) return DGM_BAD_ARGUMENT;
-  const double hcr = c->prm.hcr;
-  const double a = std::sqrt(hcr * pressure / density);
-  const double rad = aoa_deg * (M_PI / 180.0);
-  const double u = mach * std::cos(rad) * a, v = mach * std::sin(rad) * a;
-  const double q[4] = {density, density * u, density * v, pressure / (hcr - 1.0) + 0.5 * density * (u * u + v * v)};
-  const Tables& T = c->T;
-  for (int64_t e = 0; e < c->M.ne; ++e) {
-    const double hj = 0.5 * c->M.el_J[e];
-    for (int var = 0; var < 4; ++var)
-      for (int k = 0; k < T.np; ++k) {
-        double s = 0.0;
-        for (int i = 0; i < T.nq; ++i) s += hj * T.cell_w[i] * q[var] * T.phi_cell[i * T.np + k];
-        U[(e * 4 + var) * T.np + k] = s / (T.mass[k] * hj);
-      }
-  }
+  initial_state(c->T, c->M, density, aoa_deg, mach, pressure, c->prm.hcr, U);
   return DGM_OK;
 }
 

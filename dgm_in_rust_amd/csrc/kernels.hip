@@ -665,11 +665,11 @@ __device__ __forceinline__ double read_dt_slot(const unsigned long long* slot_ba
 // stand-alone time-step reduction into the slot of the step about to begin
 // (one wavefront = one 64-element block = one case; every lane of it reaches the reduction)
 template <class T, int NP>
-__global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, Ctl ctl, int force) {
+__global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, Ctl ctl, int force, int only) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if ((e >> 6) >= (m.nes >> 6)) return;
   const int cs = m.el_case64[e >> 6];
-  if (cs < 0) return;
+  if (cs < 0 || (only >= 0 && cs != only)) return;
   const CaseCtl& cc = ctl.cases[cs];
   if (!force && !cc.live) return;
   double dt = 1.0e10;
@@ -1380,10 +1380,10 @@ class SolverImpl final : public Solver {
     if (n > 0) k_smoothness<P, NQ, NE><<<n, 64, 0, st>>>(s.mesh, s.ltab, s.smat, e0);
   }
 
-  void time_step(const DeviceState& s, hipStream_t st, int force) const override {
+  void time_step(const DeviceState& s, hipStream_t st, int force, int only) const override {
     if (s.mesh.ne <= 0) return;
     const T* U = sizeof(T) == 8 ? (const T*)s.un : (const T*)s.sn;
-    k_dt<T, NP><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ctl, force);
+    k_dt<T, NP><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.mesh, U, s.ctl, force, only);
   }
 
   void upload(const DeviceState& s, const double* host_layout_dev, int e0, int n, hipStream_t st) const override {

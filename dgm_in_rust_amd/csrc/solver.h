@@ -55,8 +55,8 @@ class Solver {
   // per-element smoothness matrices of the WENO indicator (set-up), elements e0 .. e0 + n - 1
   virtual int smoothness_len() const = 0;
   virtual void build_smoothness(const DeviceState& s, int e0, int n, hipStream_t st) const = 0;
-  // time-step reduction into the dt slot (step + pending) of every live case (force: also inactive ones)
-  virtual void time_step(const DeviceState& s, hipStream_t st, int force) const = 0;
+  // time-step reduction into the dt slot (step + pending) of every live case, or of case `only` when >= 0
+  virtual void time_step(const DeviceState& s, hipStream_t st, int force, int only = -1) const = 0;
   // host layout [n][4][Np] (device copy) <-> state arrays, elements e0 .. e0 + n - 1
   virtual void upload(const DeviceState& s, const double* host_layout_dev, int e0, int n, hipStream_t st) const = 0;
   virtual void download(const DeviceState& s, const double* soa, double* host_layout_dev, int e0, int n,

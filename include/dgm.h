@@ -171,6 +171,67 @@ int dgm_troubled_counts(dgm_ctx* ctx, int32_t counts[4]);
 /* Lift coefficient Cl = (Fy cos a - Fx sin a) / (0.5 rho V^2 c) of the wall pressure (extension), chord c. */
 int dgm_lift_coefficient(dgm_ctx* ctx, double chord, double* cl);
 
+/* ---- batches: many cases of uneven mesh size and order advanced together (extension) ----
+ * The reference runs one case per process (src/main.rs, Solver::solve src/solver.rs:80-82); these entry
+ * points have no counterpart there. A batch keeps one arena per order p: every case is a disconnected,
+ * 64-aligned piece of that arena's element / edge arrays, so each phase of a step is one launch per
+ * order over all of its cases (surface, element, detector, rebuild, apply), with per-case time steps,
+ * stop conditions and error status. Cases are added and removed between steps (continuous batching);
+ * every case gives the same result alone as in any mix. */
+typedef struct dgm_batch dgm_batch;
+typedef struct {
+  int32_t precision;      /* DGM_FP64 / DGM_FP32_BULK */
+  int32_t limiter;        /* DGM_LIMITER_OFF or DGM_LIMITER_JACOBI */
+  double hcr;             /* ratio of specific heats of every case */
+  int32_t max_cases;      /* case slots */
+  int32_t pad;
+  int64_t capacity[4];    /* element capacity of the arena of order p = index + 1 (0: order not served) */
+} dgm_batch_params;
+typedef struct {
+  int32_t order;          /* 1..4 with the default rules */
+  int32_t final_step;     /* stop after this many steps ... */
+  double final_time;      /* ... or at this time (TemperalDisc::time_march, src/temporal_disc.rs:21-57) */
+  double cfl;             /* <= 0: the reference formula of the order */
+  double fs_mach, fs_aoa_deg, fs_rho, fs_p, fs_hcr;  /* far-field and initial state */
+} dgm_case_params;
+typedef struct {
+  int32_t live, active;   /* slot holds a case; the case is still stepping */
+  int32_t status;         /* dgm_status of the case (DGM_OK while clean) */
+  int32_t steps;
+  double time, last_dt;
+  int64_t error_edge;     /* case-local edge of a failure, or -1 */
+  int32_t troubled_last_stage, order;
+  int64_t n_elements, footprint_bytes;
+} dgm_case_info;
+typedef struct {
+  int32_t surface_block, element_waves, element_block, detect_block, remap, weno_grid;
+} dgm_tuning;
+
+int dgm_batch_create(const dgm_batch_params* params, dgm_batch** out);
+void dgm_batch_destroy(dgm_batch* b);
+int dgm_batch_last_error(const dgm_batch* b, dgm_error* err);
+/* device bytes a case of this size takes in the batch (its admission footprint) */
+int dgm_batch_footprint(const dgm_batch* b, int32_t order, int64_t n_elements, int64_t n_edges, int64_t n_patches,
+                        int64_t* bytes);
+/* Admits a case: initial state U0 in the reference layout [ne][4][Np] (NULL: the uniform far-field state,
+ * Solver::set_initial_solution src/solver.rs:45-79). DGM_NO_SPACE when its arena has no free range now. */
+int dgm_batch_add(dgm_batch* b, const dgm_mesh* mesh, const dgm_case_params* params, const double* U0,
+                  int32_t* case_id);
+int dgm_batch_remove(dgm_batch* b, int32_t case_id);
+/* enqueues n steps of every running case (one captured graph per step); returns at once */
+int dgm_batch_steps_async(dgm_batch* b, int32_t n_steps);
+int dgm_batch_synchronize(dgm_batch* b);
+/* control records of slots 0 .. n-1 (one device read) */
+int dgm_batch_case_info(dgm_batch* b, int32_t n, dgm_case_info* out);
+int dgm_batch_case_state(dgm_batch* b, int32_t case_id, double* U);
+/* lift coefficient of every slot 0 .. n-1 (NaN for empty slots), chord c */
+int dgm_batch_lift(dgm_batch* b, double chord, int32_t n, double* cl);
+int dgm_batch_get_tuning(const dgm_batch* b, int32_t order, dgm_tuning* t);
+int dgm_batch_set_tuning(dgm_batch* b, int32_t order, const dgm_tuning* t);
+/* Times the launch shapes of every served order on the resident cases (stage-0 passes only, which
+ * leave every case's state and clock unchanged) and keeps the fastest; chosen[p - 1] reports them. */
+int dgm_batch_autotune(dgm_batch* b, int32_t repeats, dgm_tuning chosen[4]);
+
 #ifdef __cplusplus
 }
 #endif

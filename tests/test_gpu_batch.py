@@ -1,0 +1,152 @@
+"""GPU: batches of cases (dgm_batch_*). Each case in a batch must give the same state, bit for bit,
+as the same case alone in a single-case context: the per-element arithmetic does not depend on where
+a case sits in its arena or on which cases share a launch, and the time-step minimum is order
+independent. Also: per-case stop conditions, admission by free capacity (NO_SPACE), removal and
+re-use of freed ranges between steps, per-case errors, lift, and the autotuner leaving states alone."""
+import numpy as np
+import pytest
+
+from dgm_in_rust_amd import Batch, Context, SpatialDisc, TemperalDisc, meshgen
+from dgm_in_rust_amd.batch import n_basis
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (surface points, layers, seed, order, mach, aoa)
+    (40, 12, 1, 1, 0.5, 2.0),
+    (64, 16, 2, 2, 0.8, 1.25),
+    (48, 10, 3, 3, 0.7, -2.0),
+    (32, 8, 4, 2, 0.3, 5.0),
+]
+
+
+def alone(mesh, order, mach, aoa, steps, precision):
+    ctx = Context(mesh, order=order, mach=mach, aoa_deg=aoa, precision=precision, limiter="jacobi")
+    U = ctx.initial_solution(1.0, aoa, mach, 1.0e5)
+    st = TemperalDisc(ctx, final_step=steps, final_time=1.0e300).time_march(SpatialDisc(ctx), ctx.zeros(), U)
+    return U, st, ctx
+
+
+def make_batch(precision, cap=40000, max_cases=16):
+    return Batch({1: cap, 2: cap, 3: cap}, precision=precision, limiter="jacobi", max_cases=max_cases)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_matches_alone_bitwise(precision):
+    steps = 25
+    b = make_batch(precision)
+    ids = []
+    for ns, nl, seed, p, M, a in CASES:
+        ids.append(b.add(meshgen.naca0012_ogrid(ns, nl, seed=seed), p, M, a, final_step=steps))
+    assert None not in ids
+    b.steps_async(steps + 3)   # extra steps are no-ops for finished cases
+    b.synchronize()
+    info = b.info(len(CASES))
+    for cid, (ns, nl, seed, p, M, a) in zip(ids, CASES):
+        U, st, _ = alone(meshgen.naca0012_ogrid(ns, nl, seed=seed), p, M, a, steps, precision)
+        assert info[cid]["steps"] == st["steps"] == steps and info[cid]["status"] == 0
+        assert info[cid]["time"] == st["time"]
+        assert np.array_equal(b.state(cid), U), (cid, np.abs(b.state(cid) - U).max())
+    b.close()
+
+
+def test_continuous_batching_add_remove_midrun():
+    """A case admitted while others run, and one placed into a range freed by a removal, still match
+    their stand-alone runs exactly."""
+    b = make_batch("fp32", cap=6000, max_cases=8)
+    mA = meshgen.naca0012_ogrid(48, 20, seed=11)   # 1920 triangles
+    mB = meshgen.naca0012_ogrid(40, 16, seed=12)
+    mC = meshgen.naca0012_ogrid(40, 20, seed=13)
+    a = b.add(mA, 2, 0.6, 1.0, final_step=40)
+    b.steps_async(10)
+    bb = b.add(mB, 2, 0.7, 3.0, final_step=30)
+    b.steps_async(10)
+    b.synchronize()
+    assert b.info(8)[a]["steps"] == 20 and b.info(8)[bb]["steps"] == 10
+    UA_mid = b.state(a)
+    b.remove(a)
+    c = b.add(mC, 2, 0.5, -1.0, final_step=15)
+    assert c is not None
+    b.steps_async(25)
+    b.synchronize()
+    ref_A, _, _ = alone(mA, 2, 0.6, 1.0, 20, "fp32")
+    ref_B, _, _ = alone(mB, 2, 0.7, 3.0, 30, "fp32")
+    ref_C, _, _ = alone(mC, 2, 0.5, -1.0, 15, "fp32")
+    assert np.array_equal(UA_mid, ref_A)
+    assert np.array_equal(b.state(bb), ref_B)
+    assert np.array_equal(b.state(c), ref_C)
+    info = b.info(8)
+    assert info[bb]["steps"] == 30 and info[c]["steps"] == 15 and info[bb]["active"] == 0
+    b.close()
+
+
+def test_admission_by_capacity():
+    b = Batch({2: 4096}, precision="fp32", limiter="jacobi", max_cases=4)
+    m = meshgen.naca0012_ogrid(40, 20, seed=21)   # 1600 triangles -> 25 blocks of 64
+    ids = [b.add(m, 2, 0.5, 0.0) for _ in range(3)]
+    assert ids[0] is not None and ids[1] is not None and ids[2] is None   # 3 x 1600 > 4096
+    hc = Context(m, order=2, host_only=True)
+    npatch = len(m.eb_type)
+    fp = b.footprint(2, hc.info.n_elements, hc.info.n_edges, npatch)
+    assert hc.info.n_elements == 1600
+    assert b.info(4)[ids[0]]["footprint_bytes"] == fp > 1600 * 100
+    b.remove(ids[0])
+    assert b.add(m, 2, 0.5, 0.0) is not None
+    with pytest.raises(Exception):
+        b.add(m, 3, 0.5, 0.0)   # no arena for p = 3
+    b.close()
+
+
+def test_per_case_error_is_isolated():
+    """A case whose state fails the positivity check stops with its own status; the others go on
+    and match their stand-alone runs."""
+    b = make_batch("fp64", cap=8000, max_cases=4)
+    good = meshgen.naca0012_ogrid(40, 12, seed=31)
+    bad = meshgen.naca0012_ogrid(32, 8, seed=32)
+    g = b.add(good, 2, 0.6, 1.0, final_step=12)
+    ctx = Context(bad, order=2, mach=0.5, aoa_deg=0.0, limiter="jacobi")
+    Ub = ctx.initial_solution(1.0, 0.0, 0.5, 1.0e5)
+    Ub[5, 3, 0] = -1.0e5   # negative energy -> negative pressure at element 5
+    x = b.add(bad, 2, 0.5, 0.0, U0=Ub, final_step=12)
+    b.steps_async(12)
+    b.synchronize()
+    info = b.info(4)
+    assert info[x]["status"] == 2 and info[x]["active"] == 0
+    assert info[g]["status"] == 0 and info[g]["steps"] == 12
+    ref, _, _ = alone(good, 2, 0.6, 1.0, 12, "fp64")
+    assert np.array_equal(b.state(g), ref)
+    b.close()
+
+
+def test_batch_lift_matches_context():
+    b = make_batch("fp64", cap=8000, max_cases=4)
+    cases = [(meshgen.naca0012_ogrid(40, 12, seed=41), 2, 0.6, 2.0), (meshgen.naca0012_ogrid(48, 10, seed=42), 3, 0.5, -3.0)]
+    ids = [b.add(m, p, M, a, final_step=8) for m, p, M, a in cases]
+    b.steps_async(8)
+    b.synchronize()
+    cl = b.lift(1.0, 4)
+    assert np.isnan(cl[2]) and np.isnan(cl[3])
+    for cid, (m, p, M, a) in zip(ids, cases):
+        _, _, ctx = alone(m, p, M, a, 8, "fp64")
+        assert abs(cl[cid] - ctx.lift_coefficient()) <= 1e-12 * max(1.0, abs(cl[cid]))
+    b.close()
+
+
+def test_autotune_leaves_states_unchanged(tmp_path):
+    steps = 12
+    b = make_batch("fp32")
+    ids = [b.add(meshgen.naca0012_ogrid(ns, nl, seed=seed), p, M, a, final_step=steps)
+           for ns, nl, seed, p, M, a in CASES]
+    b.steps_async(5)
+    chosen = b.autotune(repeats=3, cache=str(tmp_path / "tune.json"))
+    assert set(chosen) == {1, 2, 3}
+    for p, t in chosen.items():
+        assert t["surface_block"] in (64, 128, 256) and t["element_waves"] in (1, 2, 4)
+        assert b.tuning(p) == t
+    again = b.autotune(cache=str(tmp_path / "tune.json"))   # served from the cache file
+    assert again == chosen
+    b.steps_async(steps)
+    b.synchronize()
+    for cid, (ns, nl, seed, p, M, a) in zip(ids, CASES):
+        U, _, _ = alone(meshgen.naca0012_ogrid(ns, nl, seed=seed), p, M, a, steps, "fp32")
+        assert np.array_equal(b.state(cid), U)
+    b.close()
