@@ -5,6 +5,7 @@
   with the mean flow. Not part of the reference (which has no periodic boundary); it pins the
   whole scheme's accuracy.
 * Uniform freestream (the reference's initial condition, src/solver.rs:45-79).
+* The batched-sweep distribution of airfoil cases (:func:`sweep_cases`).
 
 Norms are evaluated with the cell quadrature rule of the context: for a field q given at the
 cell points, ||q||_2^2 = sum_e sum_q (J_e / 2) w_q q^2.
@@ -67,3 +68,24 @@ def relative_l2(ctx, U, Uref):
     d = l2_norm_of_points(ctx, point_values(ctx, U - Uref))
     r = l2_norm_of_points(ctx, point_values(ctx, Uref))
     return float(np.sqrt((d * d).sum() / (r * r).sum()))
+
+
+def ogrid_shape(n_triangles, ratio):
+    """(surface points, layers) of an O-grid with about n_triangles = 2 * ns * nl and ns / nl = ratio."""
+    nl = max(8, int(round(np.sqrt(n_triangles / (2.0 * ratio)))))
+    ns = max(16, int(round(n_triangles / (2.0 * nl))))
+    return ns, nl
+
+
+def sweep_cases(n, seed=0, tri_min=2000, tri_max=60000, orders=(1, 2, 3), mach=(0.3, 0.85), aoa=(-4.0, 6.0)):
+    """n airfoil cases: mesh size log-uniform in [tri_min, tri_max] triangles (NACA 0012 O-grid with a
+    random surface / layer split, ns / nl uniform in [3, 6.5]), order uniform over ``orders``, Mach and
+    angle of attack uniform. Returns dicts with ns, nl, mesh_seed, order, mach, aoa_deg."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        tri = float(np.exp(rng.uniform(np.log(tri_min), np.log(tri_max))))
+        ns, nl = ogrid_shape(tri, rng.uniform(3.0, 6.5))
+        out.append(dict(ns=ns, nl=nl, mesh_seed=int(rng.integers(1 << 30)), order=int(rng.choice(orders)),
+                        mach=float(rng.uniform(*mach)), aoa_deg=float(rng.uniform(*aoa))))
+    return out

@@ -167,3 +167,19 @@ def test_bad_meshes_rejected():
     with pytest.raises(DgmError) as e:
         Context(m, order=2, n_cell_gp=7, host_only=True)
     assert e.value.status == 4
+
+
+def test_sweep_case_distribution():
+    from dgm_in_rust_amd import cases
+    from dgm_in_rust_amd.serving import poisson_arrivals
+    spec = cases.sweep_cases(400, seed=3)
+    tri = np.array([2 * c["ns"] * c["nl"] for c in spec])
+    assert tri.min() >= 1500 and tri.max() <= 66000
+    # log-uniform: the median sits near the geometric mean of the range
+    assert abs(np.log(np.median(tri)) - 0.5 * (np.log(2000) + np.log(60000))) < 0.35
+    assert {c["order"] for c in spec} == {1, 2, 3}
+    assert all(0.3 <= c["mach"] <= 0.85 and -4.0 <= c["aoa_deg"] <= 6.0 for c in spec)
+    m = meshgen.naca0012_ogrid(spec[0]["ns"], spec[0]["nl"], seed=spec[0]["mesh_seed"])
+    assert int(m.el_count.sum()) == tri[0]
+    t = poisson_arrivals(8.0, 4000, seed=1)
+    assert np.all(np.diff(t) > 0) and abs(4000 / t[-1] - 8.0) < 0.5
