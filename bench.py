@@ -17,6 +17,15 @@ bytes per dispatch that rocprofv3's FETCH_SIZE / WRITE_SIZE counters measured fo
 read from the workload's committed summary under profiles/ (null when there is none).
 The CPU baseline times the oracle (FP64 C++ restatement of the reference, OpenMP in the three
 residual folds like the reference's rayon folds) on a few steps of the same case.
+
+Also measured in the same run (sub-objects of the JSON line):
+* sweep: 256 airfoil cases (log-uniform 2k-60k triangles, p in {1, 2, 3}, M in [0.3, 0.85], AoA in
+  [-4, 6] deg) advanced together in one batch for 1000 steps each, FP32-bulk, autotuned launch shapes;
+  aggregate GDOF * stage / s.
+* serving: Poisson arrivals at 2, 8 and 32 requests/s of cases from the same distribution, each run
+  until its lift coefficient changes by less than 1e-5 over 100 steps or for 20 000 steps; p50 / p99
+  time-to-converged-lift (arrival to exit), throughput and peak concurrency, FP32-bulk at every rate and
+  FP64 at the middle rate.
 """
 import argparse
 import json
@@ -120,6 +129,59 @@ def attach_traffic(summ, r, args, fp32):
         summ["roofline"]["traffic_source"] = f"profiles/{name}"
 
 
+def sweep(ncases, steps, seed=0):
+    """Batched sweep: ncases airfoil cases (log-uniform 2k-60k triangles, p in {1, 2, 3}, M in [0.3, 0.85],
+    AoA in [-4, 6] deg) in one batch, FP32-bulk, autotuned launch shapes, each case `steps` steps from
+    freestream. Aggregate GDOF * stage / s over the steps the cases actually ran."""
+    from dgm_in_rust_amd import Batch, cases, meshgen
+    from dgm_in_rust_amd.batch import n_basis
+    spec = cases.sweep_cases(ncases, seed=seed)
+    meshes = [meshgen.naca0012_ogrid(c["ns"], c["nl"], seed=c["mesh_seed"]) for c in spec]
+    cap = {p: 0 for p in (1, 2, 3)}
+    for c, m in zip(spec, meshes):
+        cap[c["order"]] += (int(m.el_count.sum()) + 63) // 64 * 64
+    b = Batch(cap, precision="fp32", limiter="jacobi", max_cases=ncases)
+    ids = [b.add(m, c["order"], c["mach"], c["aoa_deg"], final_step=steps) for c, m in zip(spec, meshes)]
+    tuned = b.autotune(repeats=5)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.steps_async(steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    info = b.info(ncases)
+    dofst = sum(int(m.el_count.sum()) * 4 * n_basis(c["order"]) * 3 * info[i]["steps"]
+                for i, c, m in zip(ids, spec, meshes))
+    b.close()
+    ne = [int(m.el_count.sum()) for m in meshes]
+    return dict(value=dofst / wall / 1e9, unit="GDOF·stage/s", cases=ncases, steps=steps, wall_s=wall,
+                ms_per_step=1e3 * wall / steps, elements=int(sum(ne)), elements_min=min(ne), elements_max=max(ne),
+                per_order_elements={str(p): v for p, v in cap.items()},
+                finished_ok=int(sum(1 for i in ids if info[i]["status"] == 0 and info[i]["steps"] == steps)),
+                failed=int(sum(1 for i in ids if info[i]["status"] != 0)), tuning={str(p): t for p, t in tuned.items()},
+                precision="fp32_bulk", limiter="jacobi", seed=seed)
+
+
+def serve(rate, nreq, precision, max_steps=20000, cl_tol=1e-5, seed=0, time_limit=60.0):
+    """Serving loop at Poisson rate `rate`/s over nreq cases of the sweep distribution; each runs until
+    |dCl| < cl_tol over 100 steps or max_steps. Returns p50/p99 time-to-converged-lift and throughput."""
+    from dgm_in_rust_amd import Batch, cases, meshgen
+    from dgm_in_rust_amd.serving import Request, Server, poisson_arrivals, summary
+    spec = cases.sweep_cases(nreq, seed=seed)
+    arr = poisson_arrivals(rate, nreq, seed=seed + 1)
+    reqs = [Request(i, float(t), meshgen.naca0012_ogrid(c["ns"], c["nl"], seed=c["mesh_seed"]), c["order"], c["mach"],
+                    c["aoa_deg"]) for i, (c, t) in enumerate(zip(spec, arr))]
+    cap = 4_000_000
+    with Batch({1: cap, 2: cap, 3: cap}, precision=precision, limiter="jacobi", max_cases=1024) as b:
+        srv = Server(b, cl_tol=cl_tol, max_steps=max_steps)
+        t0 = time.perf_counter()
+        res = srv.run(reqs, time_limit=time_limit)
+        wall = time.perf_counter() - t0
+    out = summary(res, srv, wall)
+    out.update(rate_per_s=rate, requests=nreq, max_steps=max_steps, cl_tol=cl_tol, time_limit_s=time_limit,
+               unfinished=nreq - len(res))
+    return out
+
+
 def cpu_baseline(mesh, order, mach, aoa, nsteps):
     from oracle.refcpu import RefCase
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -145,6 +207,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--sweep-cases", type=int, default=256, help="batched sweep size (0: skip)")
+    ap.add_argument("--sweep-steps", type=int, default=1000)
+    ap.add_argument("--serve-requests", type=int, default=16, help="requests per serving rate (0: skip)")
+    ap.add_argument("--serve-rates", default="2,8,32")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (counter scale reference for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -177,6 +243,21 @@ def main():
     attach_traffic(s32, r32, args, True)
     if s64:
         attach_traffic(s64, r64, args, False)
+    sw = None
+    if args.sweep_cases > 0:
+        sw = sweep(args.sweep_cases, args.sweep_steps)
+        print(f"sweep: {sw['value']:.2f} GDOF*stage/s over {sw['cases']} cases ({sw['elements']} elements), "
+              f"{sw['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    sv = None
+    if args.serve_requests > 0:
+        sv = {"fp32_bulk": {}, "fp64": {}}
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        for r in rates:
+            sv["fp32_bulk"][f"{r:g}"] = serve(r, args.serve_requests, "fp32")
+            print(f"serve fp32 rate {r:g}: {sv['fp32_bulk'][f'{r:g}']}", file=sys.stderr, flush=True)
+        mid = rates[len(rates) // 2] if rates else 8.0
+        sv["fp64"][f"{mid:g}"] = serve(mid, args.serve_requests, "fp64")
+        print(f"serve fp64 rate {mid:g}: {sv['fp64'][f'{mid:g}']}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args.order, mach, aoa, args.cpu_steps)
@@ -195,6 +276,8 @@ def main():
         "kernel_ms": s32["kernel_ms"],
         "troubled_fraction": s32["troubled_fraction"],
         "cpu_baseline": cpu,
+        "sweep": sw,
+        "serving": sv,
     }
     if s64:
         line["fp64"] = {"value": s64["value"], "ms_per_step": s64["ms_per_step"], "roofline": s64["roofline"],

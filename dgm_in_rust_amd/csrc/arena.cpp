@@ -293,6 +293,7 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
   dm.patch_q = pq;
   dm.el_case64 = elc;
   dm.ed_case64 = edc;
+  dm.single_case = -1;
 
   // state and work buffers
   const size_t ns = (size_t)4 * np * cap_e;

@@ -185,6 +185,7 @@ int build_device(dgm_ctx* c) {
   const std::vector<double> pq = farfield_states(c->M, P.fs_mach, P.fs_aoa_deg, P.fs_rho, P.fs_p, P.fs_hcr, P.hcr);
   if (int s = A.place(c->M, pq, 0, c->pl, e)) return fail(c, s, e.empty() ? "mesh does not fit its arena" : e);
   A.update_extent();
+  A.S.mesh.single_case = 0;  // the one case owns every block of this arena
   std::vector<int> lv(c->M.level_elems.begin(), c->M.level_elems.end());
   if (int s = dalloc(c, &c->d_levels, lv.size())) return s;
   if (!lv.empty()) HIPCHK(c, hipMemcpy(c->d_levels, lv.data(), lv.size() * 4, hipMemcpyHostToDevice));
@@ -534,36 +535,52 @@ int dgm_device_clock(dgm_ctx* c, double* t, int32_t* steps, double* last_dt) {
 int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   if (!c || !c->stream || !out || n < 1) return DGM_BAD_ARGUMENT;
   std::memset(out, 0, sizeof(*out));
-  const int nev = 3 + 3 * Solver::N_PARTS;
-  std::vector<hipEvent_t> ev(nev);
+  constexpr int NP_ = Solver::N_PARTS;
+  // per part: a start and a stop event; control: 3 stream events
+  std::vector<hipEvent_t> ev(2 * 3 * NP_ + 3);
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
-  double acc[Solver::N_PARTS] = {0, 0, 0, 0, 0};
+  DeviceState& S = dev(c);
+  const int lim = c->prm.limiter;
+  double acc[NP_] = {0, 0, 0, 0, 0};
   double t_dt = 0.0, t_ctl = 0.0, t_step = 0.0;
   for (int it = 0; it < n; ++it) {
-    int k = 0;
-    HIPCHK(c, hipEventRecord(ev[k++], c->stream));
-    if (!fused_dt(c)) solver(c).time_step(dev(c), c->stream, 0);
-    HIPCHK(c, hipEventRecord(ev[k++], c->stream));
-    launch_step_begin(dev(c).ctl, 1, c->stream);
-    HIPCHK(c, hipEventRecord(ev[k++], c->stream));
+    hipEvent_t* ctl_ev = &ev[2 * 3 * NP_];
+    HIPCHK(c, hipEventRecord(ctl_ev[0], c->stream));
+    if (!fused_dt(c)) solver(c).time_step(S, c->stream, 0);
+    HIPCHK(c, hipEventRecord(ctl_ev[1], c->stream));
+    launch_step_begin(S.ctl, 1, c->stream);
+    HIPCHK(c, hipEventRecord(ctl_ev[2], c->stream));
     for (int st = 0; st < 3; ++st)
-      for (int part = 0; part < Solver::N_PARTS; ++part) {
-        solver(c).stage_part(dev(c), st, part, c->stream);
-        HIPCHK(c, hipEventRecord(ev[k++], c->stream));
+      for (int part = 0; part < NP_; ++part) {
+        hipEvent_t* pe = &ev[2 * (st * NP_ + part)];
+        // one launch per part (surface, element, and every part of the Jacobi limiter): the dispatch
+        // records its own start and end; otherwise stream events bracket the part
+        const bool single = part <= Solver::PART_ELEMENT || lim == DGM_LIMITER_JACOBI;
+        if (single) {
+          S.prof_ev[0] = pe[0];
+          S.prof_ev[1] = pe[1];
+          solver(c).stage_part(S, st, part, c->stream);
+          S.prof_ev[0] = S.prof_ev[1] = nullptr;
+        } else {
+          HIPCHK(c, hipEventRecord(pe[0], c->stream));
+          solver(c).stage_part(S, st, part, c->stream);
+          HIPCHK(c, hipEventRecord(pe[1], c->stream));
+        }
       }
-    HIPCHK(c, hipEventSynchronize(ev[k - 1]));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[0], ctl_ev[1]));
     t_dt += ms;
-    HIPCHK(c, hipEventElapsedTime(&ms, ev[1], ev[2]));
+    HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[1], ctl_ev[2]));
     t_ctl += ms;
-    int idx = 2;
     for (int st = 0; st < 3; ++st)
-      for (int part = 0; part < Solver::N_PARTS; ++part, ++idx) {
-        HIPCHK(c, hipEventElapsedTime(&ms, ev[idx], ev[idx + 1]));
+      for (int part = 0; part < NP_; ++part) {
+        hipEvent_t* pe = &ev[2 * (st * NP_ + part)];
+        HIPCHK(c, hipEventElapsedTime(&ms, pe[0], pe[1]));
         acc[part] += ms;
       }
-    HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[idx]));
+    HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[0], ev[2 * (2 * NP_ + NP_ - 1) + 1]));
     t_step += ms;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);

@@ -39,7 +39,7 @@ struct DevMesh {
   const int* el_case64;    // [nes / 64] case of each 64-element block, -1 unused
   const int* ed_case64;    // [edge capacity / 64] case of each 64-edge block, -1 unused
   int ed_stride;           // edge capacity (stride of ed_elem / ed_geo)
-  int pad;
+  int single_case;         // >= 0: every block belongs to this case (no map lookup); -1: use the maps
 };
 
 // Step control of one case, resident on the device so that whole steps replay from a captured graph.

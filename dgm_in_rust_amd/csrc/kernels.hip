@@ -14,6 +14,7 @@
 //              then one wavefront per troubled element for the characteristic WENO rebuild
 //              (weno.rs:86-269) with FP64 smoothness indicators and weights.
 // Per step: k_dt min-reduction (src/temporal_disc.rs:58-88) and device-side step control.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -242,6 +243,30 @@ __device__ __forceinline__ void record_error(CaseCtl* cc, int edge, int code) {
 
 __device__ __forceinline__ bool case_runs(const CaseCtl& cc) { return cc.active && cc.error == ~0ull; }
 
+// Case of a 64-block, read through the scalar unit: the block index is uniform over the wavefront
+// (every caller's wavefront covers one 64-block), so the case record that follows is a scalar load.
+__device__ __forceinline__ int case_of(const DevMesh& m, const int* __restrict__ map, int index) {
+  if (m.single_case >= 0) return m.single_case;
+  return map[__builtin_amdgcn_readfirstlane(index >> 6)];
+}
+
+// Appends the lanes with pred set to a list counter with one atomic per wavefront; returns each such
+// lane's slot (-1 for the others). Every active lane of the wavefront must call it.
+__device__ __forceinline__ int wave_append(int* counter, bool pred, int* also) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return -1;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) {
+    const int n = __popcll(mask);
+    base = atomicAdd(counter, n);
+    atomicAdd(also, n);
+  }
+  base = __shfl(base, leader, 64);
+  return pred ? base + __popcll(mask & ((1ull << lane) - 1ull)) : -1;
+}
+
 // dt slot of a case for the given step parity
 __device__ __forceinline__ unsigned long long* dt_slot(const Ctl& ctl, int c, int parity) {
   return ctl.dtmin + (size_t)c * DT_CASE + (size_t)(parity & 1) * DT_SLOT;
@@ -260,7 +285,7 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
   const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int ed = blk * blockDim.x + threadIdx.x;
   if (ed >= m.nedge) return;
-  const int cs = m.ed_case64[ed >> 6];
+  const int cs = case_of(m, m.ed_case64, ed);
   if (cs < 0) return;
   CaseCtl* cc = ctl.cases + cs;
   if (!force && !case_runs(*cc)) return;
@@ -344,7 +369,7 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int e = blk * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
-  const int cs = m.el_case64[e >> 6];
+  const int cs = case_of(m, m.el_case64, e);
   if (cs < 0) return;
   const CaseCtl& cc = ctl.cases[cs];
   if (!force && !case_runs(cc)) return;
@@ -489,7 +514,7 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
   const int lane = threadIdx.x & 63;
   const int wave = tid >> 6;
   // the wavefront's 16 elements lie in one 64-element block, hence in one case (uniform exit)
-  const int cs = m.el_case64[(wave * 16) >> 6];
+  const int cs = case_of(m, m.el_case64, wave * 16);
   if (cs < 0) return;
   const CaseCtl& cc = ctl.cases[cs];
   if (!force && !case_runs(cc)) return;
@@ -668,7 +693,7 @@ template <class T, int NP>
 __global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, Ctl ctl, int force, int only) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if ((e >> 6) >= (m.nes >> 6)) return;
-  const int cs = m.el_case64[e >> 6];
+  const int cs = case_of(m, m.el_case64, e);
   if (cs < 0 || (only >= 0 && cs != only)) return;
   const CaseCtl& cc = ctl.cases[cs];
   if (!force && !cc.live) return;
@@ -720,32 +745,33 @@ __global__ void k_reset_count(Ctl ctl, int slot) { ctl.count[slot] = 0; }
 // local edge index with reversed point order (:21-27, :63-64); faces without a neighbour are skipped;
 // no inflow face gives 0/0 = NaN, which is not >= 1.
 template <class T, int P, int NQ, int NE>
-__device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, int e,
-                      double* mode_sums = nullptr) {
+__device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, int e,
+                                      double& sum0, double& sum1, double& sum2, double& sum3) {
   using LO = LimOff<P, NQ, NE>;
   constexpr int NP = LO::NP;
   const int nes = m.nes;
-  double rho[NP], ru[NP], rv[NP], en[NP];
+  // modes held in the bulk type (half the registers in FP32-bulk), arithmetic in FP64
+  T rho[NP], ru[NP], rv[NP], en[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
-    rho[k] = (double)U[(size_t)(0 * NP + k) * nes + e];
-    ru[k] = (double)U[(size_t)(1 * NP + k) * nes + e];
-    rv[k] = (double)U[(size_t)(2 * NP + k) * nes + e];
-    en[k] = (double)U[(size_t)(3 * NP + k) * nes + e];
+    rho[k] = U[(size_t)(0 * NP + k) * nes + e];
+    ru[k] = U[(size_t)(1 * NP + k) * nes + e];
+    rv[k] = U[(size_t)(2 * NP + k) * nes + e];
+    en[k] = U[(size_t)(3 * NP + k) * nes + e];
   }
-  if (mode_sums) {
+  {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      a0 += rho[k];
-      a1 += ru[k];
-      a2 += rv[k];
-      a3 += en[k];
+      a0 += (double)rho[k];
+      a1 += (double)ru[k];
+      a2 += (double)rv[k];
+      a3 += (double)en[k];
     }
-    mode_sums[0] = a0;
-    mode_sums[1] = a1;
-    mode_sums[2] = a2;
-    mode_sums[3] = a3;
+    sum0 = a0;
+    sum1 = a1;
+    sum2 = a2;
+    sum3 = a3;
   }
   double djump = 0.0, ejump = 0.0, surface = 0.0, minr = 1.0e10, mine = 1.0e10;
   const double* phie = lt + LO::PHIE;
@@ -766,8 +792,8 @@ __device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* _
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         const double ph = phie[(lic * NE + q) * NP + k];
-        a += rho[k] * ph;
-        b += en[k] * ph;
+        a += (double)rho[k] * ph;
+        b += (double)en[k] * ph;
       }
       tr[q] = a;
       te[q] = b;
@@ -778,8 +804,8 @@ __device__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* _
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
       const double ph = phie[(lic * NE + NE / 2) * NP + k];
-      mu += ru[k] * ph;
-      mv += rv[k] * ph;
+      mu += (double)ru[k] * ph;
+      mv += (double)rv[k] * ph;
     }
     if (mu * nx + mv * ny < 0.0) {
       double rj = 0.0, ej = 0.0;
@@ -816,37 +842,36 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
   const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int e = blk * blockDim.x + threadIdx.x;
   if ((e >> 6) >= (m.nes >> 6)) return;
-  const int cs = m.el_case64[e >> 6];
+  const int cs = case_of(m, m.el_case64, e);
   if (cs < 0) return;
   CaseCtl* cc = ctl.cases + cs;
   if (!force && !case_runs(*cc)) return;
   const bool valid = e < m.ne && e < cc->e_end;
   if (!fuse_dt) {
-    if (!valid) return;
-    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e);
-    if (lb.flags) lb.flags[e] = t ? 1 : 0;
-    if (t) {
-      const int i = atomicAdd(&ctl.count[slot], 1);
-      lb.list[i] = e;
-      atomicAdd(&cc->count[slot], 1);
+    bool t = false;
+    if (valid) {
+      double s0, s1, s2, s3;
+      t = kxrcf<T, P, NQ, NE>(m, U, lt, e, s0, s1, s2, s3);
+      if (lb.flags) lb.flags[e] = t ? 1 : 0;
     }
+    const int i = wave_append(&ctl.count[slot], t, &cc->count[slot]);
+    if (i >= 0) lb.list[i] = e;
     return;
   }
+  bool t = false;
   double dt = 1.0e10;
   if (valid) {
-    double sums[4];
-    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e, sums);
+    double s0, s1, s2, s3;
+    t = kxrcf<T, P, NQ, NE>(m, U, lt, e, s0, s1, s2, s3);
     if (lb.flags) lb.flags[e] = t ? 1 : 0;
-    if (t) {
-      const int i = atomicAdd(&ctl.count[slot], 1);
-      lb.list[i] = e;
-      atomicAdd(&cc->count[slot], 1);
-    } else {
-      const double d = element_dt<1>([&](int v, int) { return sums[v]; }, m.hcr, cc->cfl,
-                                     m.el_geo[G_HMIN * m.nes + e]);
+    if (!t) {
+      const double d = element_dt<1>([&](int v, int) { return v == 0 ? s0 : (v == 1 ? s1 : (v == 2 ? s2 : s3)); },
+                                     m.hcr, cc->cfl, m.el_geo[G_HMIN * m.nes + e]);
       if (d < dt) dt = d;
     }
   }
+  const int i = wave_append(&ctl.count[slot], t, &cc->count[slot]);
+  if (i >= 0) lb.list[i] = e;
   wave_min_dt(dt, dt_slot(ctl, cs, cc->step + 1), e >> 6);
 }
 
@@ -1102,7 +1127,7 @@ __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U,
   const int n = ctl.count[slot];
   for (int s = blockIdx.x; s < n; s += gridDim.x) {
     const int e = lb.list[s];
-    const int cs = m.el_case64[e >> 6];
+    const int cs = case_of(m, m.el_case64, e);
     CaseCtl* cc = ctl.cases + cs;
     double* out = lb.w + (size_t)s * 4 * NP;
     const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, e, sh, out);
@@ -1141,12 +1166,13 @@ __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __rest
   const int s = blockIdx.x;
   if (s >= n) return;
   const int e = elems[s];
-  const int cs = m.el_case64[e >> 6];
+  const int cs = case_of(m, m.el_case64, e);
   if (cs < 0) return;
   CaseCtl* cc = ctl.cases + cs;
   if (!force && !case_runs(*cc)) return;
   if (threadIdx.x == 0) {
-    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e);
+    double s0, s1, s2, s3;
+    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e, s0, s1, s2, s3);
     sh.flag = t ? 1 : 0;
     if (lb.flags) lb.flags[e] = t ? 1 : 0;
   }
@@ -1213,7 +1239,7 @@ __global__ void k_wall_force(DevMesh m, const double* __restrict__ U, const doub
   const int ed = ed0 + i;
   const int w0 = ed0 + (int)(blockIdx.x * blockDim.x) + (int)(threadIdx.x & ~63u);  // first edge of the wavefront
   if (w0 >= m.ed_stride) return;
-  const int cs = m.ed_case64[w0 >> 6];
+  const int cs = case_of(m, m.ed_case64, w0);
   if (cs < 0) return;
   const double hcr = m.hcr;
   double fx = 0.0, fy = 0.0;
@@ -1243,6 +1269,17 @@ __global__ void k_wall_force(DevMesh m, const double* __restrict__ U, const doub
 }
 
 inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// One launch of a stage part. With the arena's profiling events set, the launch records them at the
+// dispatch's own start and end (hipExtLaunchKernel), so their elapsed time is the kernel duration
+// without the launch gaps a stream event pair would include.
+template <typename... KArgs, typename... Args>
+void go(const DeviceState& s, void (*kernel)(KArgs...), dim3 grid, dim3 block, hipStream_t st, Args... args) {
+  if (s.prof_ev[0])
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, st, s.prof_ev[0], s.prof_ev[1], 0, static_cast<KArgs>(args)...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, st, static_cast<KArgs>(args)...);
+}
 
 }  // namespace
 
@@ -1289,8 +1326,8 @@ class SolverImpl final : public Solver {
   void surface(const DeviceState& s, const T* in, hipStream_t st, int force) const {
     if (s.mesh.nedge <= 0) return;
     const int bs = s.tune.surface_block;
-    k_surface<T, NP, NQ, NE><<<blocks_for(s.mesh.nedge, bs), bs, 0, st>>>(s.mesh, in, (T*)s.flux, (const T*)s.tab,
-                                                                         s.ctl, force, s.tune.remap);
+    go(s, k_surface<T, NP, NQ, NE>, blocks_for(s.mesh.nedge, bs), bs, st, s.mesh, in, (T*)s.flux, (const T*)s.tab,
+       s.ctl, force, s.tune.remap);
   }
   template <int OP>
   void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force) const {
@@ -1298,14 +1335,14 @@ class SolverImpl final : public Solver {
     if constexpr (sizeof(T) == 4 && NP <= 16 && NQ <= 16 && 3 * NE <= 16) {
       if (s.use_mfma) {
         const int w = s.tune.element_waves;
-        k_element_mfma<NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, 16 * w), 64 * w, 0, st>>>(
-            s.mesh, b, (const float*)s.flux, (const float*)s.tab, s.ctl, force, s.tune.remap);
+        go(s, k_element_mfma<NP, NQ, NE, OP>, blocks_for(s.mesh.ne, 16 * w), 64 * w, st, s.mesh, b,
+           (const float*)s.flux, (const float*)s.tab, s.ctl, force, s.tune.remap);
         return;
       }
     }
     const int bs = s.tune.element_block;
-    k_element<T, NP, NQ, NE, OP><<<blocks_for(s.mesh.ne, bs), bs, 0, st>>>(s.mesh, b, (const T*)s.flux,
-                                                                          (const T*)s.tab, s.ctl, force, s.tune.remap);
+    go(s, k_element<T, NP, NQ, NE, OP>, blocks_for(s.mesh.ne, bs), bs, st, s.mesh, b, (const T*)s.flux,
+       (const T*)s.tab, s.ctl, force, s.tune.remap);
   }
 
   void residual(const DeviceState& s, hipStream_t st) const override {
@@ -1359,12 +1396,12 @@ class SolverImpl final : public Solver {
       const int fuse = (STAGE == 2 && slot == 2) ? 1 : 0;
       if (part == PART_DETECT) {
         const int bs = s.tune.detect_block;
-        k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, bs), bs, 0, st>>>(s.mesh, U, s.ltab, lb, s.ctl, slot, force,
-                                                                       fuse, s.tune.remap);
+        go(s, k_kxrcf<T, P, NQ, NE>, blocks_for(s.mesh.ne, bs), bs, st, s.mesh, U, s.ltab, lb, s.ctl, slot, force,
+           fuse, s.tune.remap);
       } else if (part == PART_REBUILD) {
-        k_weno<T, P, NQ, NE><<<s.tune.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, fuse);
+        go(s, k_weno<T, P, NQ, NE>, s.tune.weno_grid, 64, st, s.mesh, U, s.smat, lb, s.ctl, slot, fuse);
       } else if (part == PART_APPLY) {
-        k_apply<T, NP, STAGE><<<1024, 256, 0, st>>>(lb, b, s.mesh.nes, s.ctl, slot);
+        go(s, k_apply<T, NP, STAGE>, 1024, 256, st, lb, b, s.mesh.nes, s.ctl, slot);
       }
     } else if (s.limiter == DGM_LIMITER_SEQUENTIAL && part == PART_DETECT) {
       for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {

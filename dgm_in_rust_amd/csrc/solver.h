@@ -33,6 +33,8 @@ struct DeviceState {
   Tuning tune{};
   std::vector<int> level_offsets;
   const int* levels = nullptr;
+  // when set, the next single-launch stage part records these at its dispatch's start and end
+  hipEvent_t prof_ev[2] = {nullptr, nullptr};
 };
 
 class Solver {

@@ -158,8 +158,10 @@ void* dgm_stream(dgm_ctx* ctx);
 int dgm_check_state(dgm_ctx* ctx);
 /* time and step counters of the resident run */
 int dgm_device_clock(dgm_ctx* ctx, double* time, int32_t* steps, double* last_dt);
-/* Per-kernel timing: runs n whole steps on the resident state (eagerly, not from the graph) with HIP
- * events around every launch on the context stream; reports the mean milliseconds per launch. */
+/* Per-kernel timing: runs n whole steps on the resident state (eagerly, not from the graph) on the
+ * context stream and reports the mean milliseconds per launch of each stage part. A part that is one
+ * kernel launch is timed by events its dispatch records at its own start and end (hipExtLaunchKernel);
+ * a part of several launches (the element-order limiter sweep) by stream events around it. */
 typedef struct {
   double surface_ms, element_ms, detect_ms, rebuild_ms, apply_ms, time_step_ms, control_ms;
   double step_ms;  /* mean wall time of one step between its first and last event */
