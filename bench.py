@@ -76,6 +76,7 @@ def run_gpu(mesh, precision, steps, warmup, order, mach, aoa):
     if warmup:
         ctx.steps_async(warmup)
         ctx.synchronize()
+    tuning = ctx.autotune(repeats=10)   # launch shapes timed on this state (state and clock unchanged)
     t0 = time.perf_counter()
     ctx.steps_async(steps)
     ctx.synchronize()
@@ -85,7 +86,8 @@ def run_gpu(mesh, precision, steps, warmup, order, mach, aoa):
     ctx.check_state()
     kt = ctx.profile_steps(20)
     info = ctx.info
-    out = dict(ctx=ctx, wall=wall, steps_done=nsteps, t=tnow, dt=dt, kt=kt, ne=info.n_elements, nedge=info.n_edges,
+    out = dict(ctx=ctx, wall=wall, steps_done=nsteps, t=tnow, dt=dt, kt=kt, tuning=tuning, ne=info.n_elements,
+               nedge=info.n_edges,
                np=info.n_basis, nq=info.n_cell_gp, ng=info.n_edge_gp, cfl=info.cfl, troubled=troubled)
     return out
 
@@ -102,7 +104,7 @@ def summarise(r, steps, fp32):
     ach = kb[dom] / (per[dom] * 1e-3) / 1e9
     stage_ms = 1e3 * r["wall"] / steps / 3.0
     sb = stage_bytes_survey(r["np"], fp32) * r["ne"]
-    return dict(value=value, ms_per_step=1e3 * r["wall"] / steps, dominant=dom, kernel_ms=per,
+    return dict(value=value, ms_per_step=1e3 * r["wall"] / steps, dominant=dom, kernel_ms=per, tuning=r["tuning"],
                 troubled_fraction=sum(r["troubled"]) / (3.0 * r["ne"]), sim_time=r["t"], last_dt=r["dt"],
                 roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
                               traffic=None, kernel=(("k_element_mfma" if fp32 else "k_element") if dom == "element" else "k_surface"),
@@ -274,6 +276,7 @@ def main():
         "roofline": s32["roofline"],
         "stage_roofline": s32["stage"],
         "kernel_ms": s32["kernel_ms"],
+        "tuning": s32["tuning"],
         "troubled_fraction": s32["troubled_fraction"],
         "cpu_baseline": cpu,
         "sweep": sw,

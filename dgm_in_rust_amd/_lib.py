@@ -24,7 +24,7 @@ EXPORTED = ["dgm_ctx_create", "dgm_ctx_create_host", "dgm_ctx_destroy", "dgm_ctx
             "dgm_compute_residuals", "dgm_apply_limiter", "dgm_compute_time_step", "dgm_rk3", "dgm_time_march",
             "dgm_state_upload", "dgm_state_download", "dgm_device_time_march", "dgm_device_steps_async",
             "dgm_synchronize", "dgm_stream", "dgm_check_state", "dgm_device_clock", "dgm_lift_coefficient",
-            "dgm_profile_steps", "dgm_troubled_counts",
+            "dgm_profile_steps", "dgm_troubled_counts", "dgm_get_tuning", "dgm_set_tuning", "dgm_autotune",
             "dgm_batch_create", "dgm_batch_destroy", "dgm_batch_last_error", "dgm_batch_footprint", "dgm_batch_add",
             "dgm_batch_remove", "dgm_batch_steps_async", "dgm_batch_synchronize", "dgm_batch_case_info",
             "dgm_batch_case_state", "dgm_batch_lift", "dgm_batch_get_tuning", "dgm_batch_set_tuning",
@@ -141,6 +141,9 @@ def lib():
     L.dgm_lift_coefficient.argtypes = [vp, C.c_double, _f64p]
     L.dgm_troubled_counts.argtypes = [vp, _i32p]
     L.dgm_profile_steps.argtypes = [vp, C.c_int32, C.POINTER(KernelTimes)]
+    L.dgm_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.dgm_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    L.dgm_autotune.argtypes = [vp, C.c_int32, C.POINTER(Tuning)]
     L.dgm_batch_create.argtypes = [C.POINTER(BatchParams), C.POINTER(vp)]
     L.dgm_batch_destroy.argtypes = [vp]
     L.dgm_batch_destroy.restype = None

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 
 namespace dgm {
 
@@ -115,6 +116,34 @@ double default_cfl(int order, double cfl) {
   const double p = (double)order;
   return 1.0 / ((2.0 * p + 1.0) * (1.0 + 4.0 / ((p + 2.0) * (p + 2.0))));
 }
+
+Tuning to_tuning(const dgm_tuning& t) {
+  Tuning u;
+  u.surface_block = t.surface_block;
+  u.element_waves = t.element_waves;
+  u.element_block = t.element_block;
+  u.detect_block = t.detect_block;
+  u.remap = t.remap;
+  u.weno_grid = t.weno_grid;
+  return u;
+}
+dgm_tuning from_tuning(const Tuning& u) {
+  dgm_tuning t;
+  t.surface_block = u.surface_block;
+  t.element_waves = u.element_waves;
+  t.element_block = u.element_block;
+  t.detect_block = u.detect_block;
+  t.remap = u.remap;
+  t.weno_grid = u.weno_grid;
+  return t;
+}
+static bool valid_block(int v) { return v == 64 || v == 128 || v == 256; }
+bool valid_tuning(const dgm_tuning& t) {
+  return valid_block(t.surface_block) && (t.element_waves == 1 || t.element_waves == 2 || t.element_waves == 4) &&
+         valid_block(t.element_block) && valid_block(t.detect_block) && (t.remap == 0 || t.remap == 1) &&
+         t.weno_grid >= 64 && t.weno_grid <= 65536;
+}
+
 
 // ---- Arena ----
 
@@ -337,6 +366,8 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
   S.ctl.count = count;
   S.limiter = limiter;
   S.use_mfma = solver->has_mfma() ? 1 : 0;
+  // rebuild wavefronts: enough for a troubled fraction of 1 / 32 in one pass, 2048 at least
+  S.tune.weno_grid = (int)std::min<int64_t>(65536, std::max<int64_t>(2048, cap_e / 32));
   dm.hcr = 1.4;
   return DGM_OK;
 }
@@ -507,6 +538,109 @@ int Arena::download_state(const double* soa, double* U, const Placement& pl, std
 void Arena::update_extent() {
   S.mesh.ne = (int)(r_e.used_end() * 64);
   S.mesh.nedge = (int)(r_ed.used_end() * 64);
+}
+
+}  // namespace dgm
+
+namespace dgm {
+
+// The state a step reads from earlier steps: u^n (and its FP32 shadow) of every arena, the case
+// records and their dt slots. Everything else a step reads it writes first in the same step.
+struct StepSnapshot {
+  std::vector<std::pair<void*, void*>> pairs;  // (live buffer, copy)
+  std::vector<size_t> sizes;
+  ~StepSnapshot() {
+    for (auto& p : pairs) (void)hipFree(p.second);
+  }
+  int add(void* live, size_t bytes, std::string& err) {
+    void* c = nullptr;
+    ACHK(hipMalloc(&c, bytes));
+    pairs.push_back({live, c});
+    sizes.push_back(bytes);
+    return DGM_OK;
+  }
+  int copy(bool restore, hipStream_t st, std::string& err) {
+    for (size_t i = 0; i < pairs.size(); ++i) {
+      void* dst = restore ? pairs[i].first : pairs[i].second;
+      void* src = restore ? pairs[i].second : pairs[i].first;
+      ACHK(hipMemcpyAsync(dst, src, sizes[i], hipMemcpyDeviceToDevice, st));
+    }
+    return DGM_OK;
+  }
+};
+
+int autotune_steps(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned long long* dtmin, int ncases,
+                   hipStream_t st, const std::function<void()>& step, int reps, std::string& err) {
+  ACHK(hipStreamSynchronize(st));
+  StepSnapshot snap;
+  for (Arena* A : arenas) {
+    const size_t ns = (size_t)4 * A->T.np * A->cap_e;
+    if (int s = snap.add(A->S.un, ns * 8, err)) return s;
+    if (A->S.sn)
+      if (int s = snap.add(A->S.sn, ns * 4, err)) return s;
+  }
+  if (int s = snap.add(cases, sizeof(CaseCtl) * ncases, err)) return s;
+  if (int s = snap.add(dtmin, sizeof(unsigned long long) * DT_CASE * ncases, err)) return s;
+  if (int s = snap.copy(false, st, err)) return s;
+  hipEvent_t e0, e1;
+  ACHK(hipEventCreate(&e0));
+  ACHK(hipEventCreate(&e1));
+  // mean ms of one step from the snapshot (one untimed step first)
+  auto trial = [&](double& ms) -> int {
+    if (int s = snap.copy(true, st, err)) return s;
+    step();
+    ACHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) step();
+    ACHK(hipEventRecord(e1, st));
+    ACHK(hipEventSynchronize(e1));
+    ACHK(hipGetLastError());
+    float x = 0.f;
+    ACHK(hipEventElapsedTime(&x, e0, e1));
+    ms = x / reps;
+    return DGM_OK;
+  };
+  const int blocks[3] = {64, 128, 256};
+  const int waves[3] = {1, 2, 4};
+  const int remaps[2] = {0, 1};
+  int status = DGM_OK;
+  for (Arena* A : arenas) {
+    if (A->extent_elements() <= 0) continue;
+    auto pick = [&](auto set, const int* vals, int n) -> int {
+      double bt = 1e30;
+      Tuning best = A->S.tune;
+      for (int i = 0; i < n; ++i) {
+        Tuning cand = best;
+        set(cand, vals[i]);
+        A->S.tune = cand;
+        double ms = 0.0;
+        if (int s = trial(ms)) return s;
+        if (ms < bt) {
+          bt = ms;
+          set(best, vals[i]);
+        }
+      }
+      A->S.tune = best;
+      return DGM_OK;
+    };
+    status = pick([](Tuning& t, int v) { t.surface_block = v; }, blocks, 3);
+    if (!status) {
+      if (A->S.use_mfma && A->solver->has_mfma())
+        status = pick([](Tuning& t, int v) { t.element_waves = v; }, waves, 3);
+      else
+        status = pick([](Tuning& t, int v) { t.element_block = v; }, blocks, 3);
+    }
+    // (the rebuild's wavefront count is not searched: the troubled-cell count it must cover grows
+    // as shocks form, so a value timed on the current state under-provisions later steps)
+    if (!status && A->limiter == DGM_LIMITER_JACOBI)
+      status = pick([](Tuning& t, int v) { t.detect_block = v; }, blocks, 3);
+    if (!status) status = pick([](Tuning& t, int v) { t.remap = v; }, remaps, 2);
+    if (status) break;
+  }
+  if (int s = snap.copy(true, st, err)) status = status ? status : s;
+  (void)hipStreamSynchronize(st);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return status;
 }
 
 }  // namespace dgm

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -71,6 +72,7 @@ class Arena {
   int download_state(const double* soa, double* U, const Placement& pl, std::string& err);
   // kernel extents = allocated extents (a captured graph must be re-captured when these change)
   void update_extent();
+
   int64_t extent_elements() const { return S.mesh.ne; }
   int64_t extent_edges() const { return S.mesh.nedge; }
 
@@ -105,6 +107,18 @@ CaseCtl fresh_case(double cfl);
 // Solver::set_initial_solution (src/solver.rs:45-79): L2 projection of a uniform state, host layout.
 void initial_state(const Tables& T, const HostMesh& M, double density, double aoa_deg, double mach, double pressure,
                    double hcr, double* U);
+
+// Chooses the launch shapes of every arena's kernels by timing whole steps (`step` enqueues one on st),
+// one knob at a time per arena. Each trial starts from a snapshot of what a step reads from earlier
+// steps (u^n and its FP32 shadow, the case records and their dt slots), which is restored at the end,
+// so every case's state and clock are left as they were.
+int autotune_steps(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned long long* dtmin, int ncases,
+                   hipStream_t st, const std::function<void()>& step, int reps, std::string& err);
+
+// launch shapes <-> their C-ABI record
+Tuning to_tuning(const dgm_tuning& t);
+dgm_tuning from_tuning(const Tuning& u);
+bool valid_tuning(const dgm_tuning& t);
 
 // CFL of the reference for order p (src/initialization.rs:128-132), or the given value when > 0.
 double default_cfl(int order, double cfl);

@@ -140,66 +140,6 @@ int case_status_code(const CaseCtl& h, int64_t* edge) {
   return DGM_HIP_ERROR;
 }
 
-Tuning to_tuning(const dgm_tuning& t) {
-  Tuning u;
-  u.surface_block = t.surface_block;
-  u.element_waves = t.element_waves;
-  u.element_block = t.element_block;
-  u.detect_block = t.detect_block;
-  u.remap = t.remap;
-  u.weno_grid = t.weno_grid;
-  return u;
-}
-dgm_tuning from_tuning(const Tuning& u) {
-  dgm_tuning t;
-  t.surface_block = u.surface_block;
-  t.element_waves = u.element_waves;
-  t.element_block = u.element_block;
-  t.detect_block = u.detect_block;
-  t.remap = u.remap;
-  t.weno_grid = u.weno_grid;
-  return t;
-}
-bool valid_block(int v) { return v == 64 || v == 128 || v == 256; }
-bool valid_tuning(const dgm_tuning& t) {
-  return valid_block(t.surface_block) && (t.element_waves == 1 || t.element_waves == 2 || t.element_waves == 4) &&
-         valid_block(t.element_block) && valid_block(t.detect_block) && (t.remap == 0 || t.remap == 1) &&
-         t.weno_grid >= 64 && t.weno_grid <= 65536;
-}
-
-// Mean milliseconds of the five stage-0 parts of one arena under launch shapes `t`. Stage-0 passes
-// read u^n (and the case's current dt) and write only stage buffers, the flux, the troubled list and
-// the stage-0 counters, so they leave every case's state and clock as they were.
-int time_stage0(dgm_batch* b, Arena& A, const Tuning& t, int reps, double ms[Solver::N_PARTS]) {
-  A.S.tune = t;
-  std::vector<hipEvent_t> ev(Solver::N_PARTS + 1);
-  for (auto& e : ev) BCHK(b, hipEventCreate(&e));
-  for (int p = 0; p < Solver::N_PARTS; ++p) ms[p] = 0.0;
-  for (int r = 0; r < reps + 1; ++r) {  // the first pass warms up
-    BCHK(b, hipEventRecord(ev[0], b->stream));
-    for (int p = 0; p < Solver::N_PARTS; ++p) {
-      A.solver->stage_part(A.S, 0, p, b->stream);
-      BCHK(b, hipEventRecord(ev[p + 1], b->stream));
-    }
-    BCHK(b, hipEventSynchronize(ev[Solver::N_PARTS]));
-    BCHK(b, hipGetLastError());
-    if (r == 0) continue;
-    for (int p = 0; p < Solver::N_PARTS; ++p) {
-      float x = 0.f;
-      BCHK(b, hipEventElapsedTime(&x, ev[p], ev[p + 1]));
-      ms[p] += x / reps;
-    }
-  }
-  for (auto& e : ev) (void)hipEventDestroy(e);
-  return DGM_OK;
-}
-
-double total(const double ms[Solver::N_PARTS]) {
-  double s = 0.0;
-  for (int p = 0; p < Solver::N_PARTS; ++p) s += ms[p];
-  return s;
-}
-
 }  // namespace
 
 extern "C" {
@@ -456,85 +396,15 @@ int dgm_batch_set_tuning(dgm_batch* b, int32_t order, const dgm_tuning* t) {
 int dgm_batch_autotune(dgm_batch* b, int32_t reps, dgm_tuning chosen[4]) {
   if (!b || reps < 1) return DGM_BAD_ARGUMENT;
   BCHK(b, hipStreamSynchronize(b->stream));
-  for (int g = 0; g < 4; ++g) {
-    if (!b->arena[g]) continue;
-    Arena& A = *b->arena[g];
-    if (A.extent_elements() > 0) {
-      Tuning best = A.S.tune;
-      double ms[Solver::N_PARTS];
-      const int blocks[3] = {64, 128, 256};
-      const int waves[3] = {1, 2, 4};
-      const int grids[5] = {256, 512, 1024, 2048, 4096};
-      // one knob at a time, each judged by the part it shapes
-      {
-        double bt = 1e30;
-        Tuning cand = best;
-        for (int v : blocks) {
-          cand.surface_block = v;
-          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
-          if (ms[Solver::PART_SURFACE] < bt) {
-            bt = ms[Solver::PART_SURFACE];
-            best.surface_block = v;
-          }
-        }
-      }
-      {
-        double bt = 1e30;
-        Tuning cand = best;
-        const bool mfma = A.S.use_mfma && A.solver->has_mfma();
-        for (int i = 0; i < 3; ++i) {
-          if (mfma)
-            cand.element_waves = waves[i];
-          else
-            cand.element_block = blocks[i];
-          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
-          if (ms[Solver::PART_ELEMENT] < bt) {
-            bt = ms[Solver::PART_ELEMENT];
-            if (mfma)
-              best.element_waves = waves[i];
-            else
-              best.element_block = blocks[i];
-          }
-        }
-      }
-      if (A.limiter == DGM_LIMITER_JACOBI) {
-        double bt = 1e30;
-        Tuning cand = best;
-        for (int v : blocks) {
-          cand.detect_block = v;
-          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
-          if (ms[Solver::PART_DETECT] < bt) {
-            bt = ms[Solver::PART_DETECT];
-            best.detect_block = v;
-          }
-        }
-        bt = 1e30;
-        cand = best;
-        for (int v : grids) {
-          cand.weno_grid = v;
-          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
-          if (ms[Solver::PART_REBUILD] < bt) {
-            bt = ms[Solver::PART_REBUILD];
-            best.weno_grid = v;
-          }
-        }
-      }
-      {
-        double bt = 1e30;
-        Tuning cand = best;
-        for (int v = 0; v < 2; ++v) {
-          cand.remap = v;
-          if (int s = time_stage0(b, A, cand, reps, ms)) return s;
-          if (total(ms) < bt) {
-            bt = total(ms);
-            best.remap = v;
-          }
-        }
-      }
-      A.S.tune = best;
-    }
-    if (chosen) chosen[g] = from_tuning(A.S.tune);
-  }
+  std::vector<Arena*> as;
+  for (int g = 0; g < 4; ++g)
+    if (b->arena[g]) as.push_back(b->arena[g].get());
+  std::string err;
+  if (int s = autotune_steps(as, b->d_cases, b->d_dtmin, b->prm.max_cases, b->stream, [b]() { enqueue_step(b); }, reps,
+                             err))
+    return fail(b, s, err);
+  for (int g = 0; g < 4; ++g)
+    if (b->arena[g] && chosen) chosen[g] = from_tuning(b->arena[g]->S.tune);
   drop_graph(b);
   return DGM_OK;
 }

@@ -621,4 +621,36 @@ int dgm_lift_coefficient(dgm_ctx* c, double chord, double* cl) {
   return DGM_OK;
 }
 
+int dgm_get_tuning(const dgm_ctx* c, dgm_tuning* t) {
+  if (!c || !t || !c->A) return DGM_BAD_ARGUMENT;
+  *t = from_tuning(c->A->S.tune);
+  return DGM_OK;
+}
+
+int dgm_set_tuning(dgm_ctx* c, const dgm_tuning* t) {
+  if (!c || !t || !c->A || !valid_tuning(*t)) return DGM_BAD_ARGUMENT;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->A->S.tune = to_tuning(*t);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  return DGM_OK;
+}
+
+int dgm_autotune(dgm_ctx* c, int32_t reps, dgm_tuning* chosen) {
+  if (!c || !c->A || reps < 1) return DGM_BAD_ARGUMENT;
+  std::string e;
+  if (int s = autotune_steps({c->A.get()}, c->d_case, c->d_dtmin, 1, c->stream, [c]() { enqueue_step(c); }, reps, e))
+    return fail(c, s, e);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  if (chosen) *chosen = from_tuning(c->A->S.tune);
+  CaseCtl h;
+  if (int s = read_ctl(c, h)) return s;
+  return ctl_status(c, h);
+}
+
 }  // extern "C"

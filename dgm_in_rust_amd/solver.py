@@ -169,6 +169,22 @@ class Context:
         self.check(self._L.dgm_troubled_counts(self.h, c))
         return list(c)
 
+    def tuning(self) -> dict:
+        t = _lib.Tuning()
+        self.check(self._L.dgm_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, values: dict):
+        t = _lib.Tuning(**{k: int(values[k]) for k, _ in _lib.Tuning._fields_})
+        self.check(self._L.dgm_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, repeats: int = 10) -> dict:
+        """Times the kernels' launch shapes on the resident state and keeps the fastest (the state and
+        the clock are left as they are)."""
+        t = _lib.Tuning()
+        self.check(self._L.dgm_autotune(self.h, repeats, C.byref(t)))
+        return t.as_dict()
+
     def lift_coefficient(self, chord=1.0):
         cl = C.c_double()
         self.check(self._L.dgm_lift_coefficient(self.h, chord, C.byref(cl)))

@@ -172,6 +172,18 @@ int dgm_profile_steps(dgm_ctx* ctx, int32_t n_steps, dgm_kernel_times* out);
 int dgm_troubled_counts(dgm_ctx* ctx, int32_t counts[4]);
 /* Lift coefficient Cl = (Fy cos a - Fx sin a) / (0.5 rho V^2 c) of the wall pressure (extension), chord c. */
 int dgm_lift_coefficient(dgm_ctx* ctx, double chord, double* cl);
+/* Launch shapes of the kernels (extension; no counterpart in the reference): threads per block of the
+ * edge kernel, wavefronts per block of the FP32 matrix-core element kernel (16 elements each), threads
+ * per block of the FP64 element kernel and of the detector, XCD-contiguous block order, and wavefronts
+ * of the WENO rebuild. */
+typedef struct {
+  int32_t surface_block, element_waves, element_block, detect_block, remap, weno_grid;
+} dgm_tuning;
+/* read, set, or choose by timing them on the resident state (dgm_state_upload first; only stage-0
+ * passes run, so the state and clock stay as they are) */
+int dgm_get_tuning(const dgm_ctx* ctx, dgm_tuning* t);
+int dgm_set_tuning(dgm_ctx* ctx, const dgm_tuning* t);
+int dgm_autotune(dgm_ctx* ctx, int32_t repeats, dgm_tuning* chosen);
 
 /* ---- batches: many cases of uneven mesh size and order advanced together (extension) ----
  * The reference runs one case per process (src/main.rs, Solver::solve src/solver.rs:80-82); these entry
@@ -205,9 +217,6 @@ typedef struct {
   int32_t troubled_last_stage, order;
   int64_t n_elements, footprint_bytes;
 } dgm_case_info;
-typedef struct {
-  int32_t surface_block, element_waves, element_block, detect_block, remap, weno_grid;
-} dgm_tuning;
 
 int dgm_batch_create(const dgm_batch_params* params, dgm_batch** out);
 void dgm_batch_destroy(dgm_batch* b);

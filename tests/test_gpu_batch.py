@@ -150,3 +150,18 @@ def test_autotune_leaves_states_unchanged(tmp_path):
         U, _, _ = alone(meshgen.naca0012_ogrid(ns, nl, seed=seed), p, M, a, steps, "fp32")
         assert np.array_equal(b.state(cid), U)
     b.close()
+
+
+def test_context_autotune_leaves_results_unchanged():
+    mesh = meshgen.naca0012_ogrid(64, 16, seed=61)
+    ref, _, _ = alone(mesh, 3, 0.8, 1.25, 10, "fp32")
+    ctx = Context(mesh, order=3, mach=0.8, aoa_deg=1.25, precision="fp32", limiter="jacobi")
+    U = ctx.initial_solution(1.0, 1.25, 0.8, 1.0e5)
+    ctx.upload(U)
+    ctx.steps_async(4)
+    ctx.synchronize()
+    t = ctx.autotune(repeats=2)
+    assert ctx.tuning() == t and t["detect_block"] in (64, 128, 256)
+    ctx.steps_async(6)
+    ctx.synchronize()
+    assert np.array_equal(ctx.download(), ref)
