@@ -243,8 +243,13 @@ def main():
         print(f"fp64: {s64['value']:.2f} GDOF*stage/s, {s64['ms_per_step']:.4f} ms/step, kernels {s64['kernel_ms']}",
               file=sys.stderr, flush=True)
     attach_traffic(s32, r32, args, True)
+    rel = None
     if s64:
         attach_traffic(s64, r64, args, False)
+        # both runs took warmup + steps + 20 profiled steps from the same state: compare them
+        from dgm_in_rust_amd import cases
+        if r32["steps_done"] == r64["steps_done"]:
+            rel = cases.relative_l2(r64["ctx"], r32["ctx"].download(), r64["ctx"].download())
     sw = None
     if args.sweep_cases > 0:
         sw = sweep(args.sweep_cases, args.sweep_steps)
@@ -287,6 +292,8 @@ def main():
                         "stage_roofline": s64["stage"], "kernel_ms": s64["kernel_ms"],
                         "troubled_fraction": s64["troubled_fraction"]}
         line["fp32_over_fp64"] = s32["value"] / s64["value"]
+        line["fp32_vs_fp64_rel_l2"] = rel
+        line["fp32_vs_fp64_steps"] = r32["steps_done"]
     print(json.dumps(line), flush=True)
 
 

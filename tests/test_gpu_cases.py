@@ -119,3 +119,27 @@ def test_ogrid_c1_short_run_matches_oracle():
     assert 0.0 < spread < 1e-10
     assert rel_l2(U, Uref) <= 1e-14
     assert rel_err(U, Uref) <= 10.0 * spread, (rel_err(U, Uref), spread)
+
+
+def test_ogrid_c1_100_steps_within_oracle_sensitivity():
+    """Config-1 march to step 100 (before any limiter decision near the KXRCF threshold flips). The
+    state is held to 1e-12 in relative L2, and in the per-variable max norm to 20 times the oracle's
+    own spread under one-ulp perturbations of its initial state. (Between steps 100 and 200 a
+    marginal troubled-cell decision flips and the trajectory becomes chaotic: the oracle's own one-ulp
+    spread grows to about 1e-1 by step 500.)"""
+    mesh = meshgen.naca0012_ogrid(160, 50, seed=0)
+    kw = dict(order=2, cfl=0.3, mach=0.5, aoa_deg=1.25)
+    ref = RefCase(mesh, limiter=1, nthreads=16, **kw)
+    ctx = Context(mesh, limiter="sequential", **kw)
+    U0 = ref.initial_solution(1.0, 1.25, 0.5, 1.0e5)
+    Uref, s, n, _ = ref.time_march(U0, 100, 1.0e9)
+    U, st = march(ctx, U0, 100, 1.0e9)
+    assert s == 0 and st["steps"] == n == 100
+    rng = np.random.default_rng(1)
+    spread = 0.0
+    for _ in range(2):
+        V = U0 * (1.0 + rng.choice([-1.0, 0.0, 1.0], size=U0.shape) * 2.0 ** -52)
+        spread = max(spread, rel_err(ref.time_march(V, 100, 1.0e9)[0], Uref))
+    assert 0.0 < spread < 1e-9
+    assert rel_l2(U, Uref) <= 1e-12
+    assert rel_err(U, Uref) <= 20.0 * spread, (rel_err(U, Uref), spread)
