@@ -99,7 +99,8 @@ def summarise(r, steps, fp32):
     kb = kernel_bytes(r["np"], r["ng"], tsize, r["ne"], r["nedge"])
     kt = r["kt"]
     per = {"element": kt["element_ms"], "surface": kt["surface_ms"], "detect": kt["detect_ms"],
-           "rebuild": kt["rebuild_ms"], "apply": kt["apply_ms"], "time_step": kt["time_step_ms"]}
+           "rebuild": kt["rebuild_ms"], "apply": kt["apply_ms"], "time_step": kt["time_step_ms"],
+           "step_control": kt["control_ms"]}
     dom = max(("element", "surface"), key=lambda k: per[k])
     ach = kb[dom] / (per[dom] * 1e-3) / 1e9
     stage_ms = 1e3 * r["wall"] / steps / 3.0

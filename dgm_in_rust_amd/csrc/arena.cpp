@@ -355,6 +355,8 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
   if (int s = dalloc(&S.lim_list, cap_e, err)) return s;
   if (int s = dalloc(&S.lim_w, (size_t)cap_e * 4 * np, err)) return s;
   if (int s = dalloc(&S.lim_flags, cap_e, err)) return s;
+  if (int s = dalloc(&S.lim_pend, cap_e, err)) return s;
+  ACHK(hipMemset(S.lim_pend, 0xff, cap_e * 4));
   if (limiter != DGM_LIMITER_OFF) {
     if (int s = dalloc(&S.smat, (size_t)cap_e * solver->smoothness_len(), err)) return s;
   }
@@ -501,6 +503,7 @@ int Arena::place(const HostMesh& M, const std::vector<double>& patch_q, int case
 
 int Arena::unplace(const Placement& pl, std::string& err) {
   DevMesh& dm = S.mesh;
+  ACHK(hipMemsetAsync(S.lim_pend + pl.e_off, 0xff, pl.ne_alloc * 4, stream));
   if (int s = set_case64(const_cast<int*>(dm.el_case64), pl.e_off / 64, pl.ne_alloc / 64, -1, err)) return s;
   if (int s = set_case64(const_cast<int*>(dm.ed_case64), pl.ed_off / 64, pl.ned_alloc / 64, -1, err)) return s;
   r_e.release(pl.e_off / 64, pl.ne_alloc / 64);
@@ -517,6 +520,8 @@ int Arena::upload_state(const double* U, const Placement& pl, std::string& err) 
   }
   ACHK(hipMemcpyAsync(hbuf_, U, n * 8, hipMemcpyHostToDevice, stream));
   solver->upload(S, hbuf_, (int)pl.e_off, (int)pl.ne, stream);
+  // nothing of an earlier state stays pending over the uploaded one
+  ACHK(hipMemsetAsync(S.lim_pend + pl.e_off, 0xff, pl.ne_alloc * 4, stream));
   ACHK(hipGetLastError());
   ACHK(hipStreamSynchronize(stream));
   return DGM_OK;
@@ -528,6 +533,7 @@ int Arena::download_state(const double* soa, double* U, const Placement& pl, std
     if (int s = dalloc(&hbuf_, n, err)) return s;
     hbuf_size_ = n;
   }
+  solver->flush(S, stream);
   solver->download(S, soa, hbuf_, (int)pl.e_off, (int)pl.ne, stream);
   ACHK(hipGetLastError());
   ACHK(hipMemcpyAsync(U, hbuf_, n * 8, hipMemcpyDeviceToHost, stream));
@@ -571,6 +577,10 @@ struct StepSnapshot {
 
 int autotune_steps(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned long long* dtmin, int ncases,
                    hipStream_t st, const std::function<void()>& step, int reps, std::string& err) {
+  // write back pending rebuilds first: the snapshot is then the whole state, and the marks the trials
+  // leave are cleared at the end
+  for (Arena* A : arenas) A->solver->flush(A->S, st);
+  ACHK(hipGetLastError());
   ACHK(hipStreamSynchronize(st));
   StepSnapshot snap;
   for (Arena* A : arenas) {
@@ -637,6 +647,8 @@ int autotune_steps(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned l
     if (status) break;
   }
   if (int s = snap.copy(true, st, err)) status = status ? status : s;
+  for (Arena* A : arenas)
+    if (hipMemsetAsync(A->S.lim_pend, 0xff, A->cap_e * 4, st) != hipSuccess && !status) status = DGM_HIP_ERROR;
   (void)hipStreamSynchronize(st);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

@@ -358,8 +358,10 @@ int dgm_batch_lift(dgm_batch* b, double chord, int32_t n, double* cl) {
   if (!b || !cl || !(chord > 0.0) || n < 0 || n > b->prm.max_cases) return DGM_BAD_ARGUMENT;
   BCHK(b, hipMemsetAsync(b->d_force, 0, 2 * sizeof(double) * b->prm.max_cases, b->stream));
   for (int g = 0; g < 4; ++g)
-    if (b->arena[g] && b->arena[g]->extent_edges() > 0)
+    if (b->arena[g] && b->arena[g]->extent_edges() > 0) {
+      b->arena[g]->solver->flush(b->arena[g]->S, b->stream);
       b->arena[g]->solver->wall_force(b->arena[g]->S, b->d_force, 0, (int)b->arena[g]->extent_edges(), b->stream);
+    }
   BCHK(b, hipGetLastError());
   std::vector<double> f(2 * (size_t)std::max(n, 1));
   if (n > 0) BCHK(b, hipMemcpyAsync(f.data(), b->d_force, 2 * sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));

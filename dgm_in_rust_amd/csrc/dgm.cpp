@@ -136,6 +136,7 @@ void enqueue_step(dgm_ctx* c) {
 
 // fresh control block for a resident run, with the first step's time step already reduced
 int start_run(dgm_ctx* c, int32_t final_step, double final_time) {
+  solver(c).flush(dev(c), c->stream);  // a resident state may still hold rebuilds of the last run
   CaseCtl h = fresh_ctl(c);
   h.final_step = final_step;
   h.final_time = final_time;
@@ -512,6 +513,7 @@ void* dgm_stream(dgm_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int dgm_check_state(dgm_ctx* c) {
   if (!c || !c->stream) return DGM_BAD_ARGUMENT;
   HIPCHK(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+  solver(c).flush(dev(c), c->stream);
   solver(c).check_finite(dev(c), c->d_flag, 0, (int)c->M.ne, c->stream);
   int f = 0;
   HIPCHK(c, hipMemcpyAsync(&f, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -536,15 +538,27 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   if (!c || !c->stream || !out || n < 1) return DGM_BAD_ARGUMENT;
   std::memset(out, 0, sizeof(*out));
   constexpr int NP_ = Solver::N_PARTS;
-  // per part: a start and a stop event; control: 3 stream events
-  std::vector<hipEvent_t> ev(2 * 3 * NP_ + 3);
+  // per part of each stage: a start and a stop event; then 4 stream events: step start, after the
+  // time-step reduction, after the step control, step end
+  std::vector<hipEvent_t> ev(2 * 3 * NP_ + 4);
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  hipEvent_t* ctl_ev = &ev[2 * 3 * NP_];
   DeviceState& S = dev(c);
   const int lim = c->prm.limiter;
+  // parts that launch nothing count zero: the Jacobi march defers its write-back, the element-order
+  // sweep is all in its detect part, and without a limiter there are no limiter parts
+  auto launches = [lim](int part) {
+    if (part <= Solver::PART_ELEMENT) return true;
+    if (lim == DGM_LIMITER_JACOBI) return part != Solver::PART_APPLY;
+    if (lim == DGM_LIMITER_SEQUENTIAL) return part == Solver::PART_DETECT;
+    return false;
+  };
+  // one launch per part (surface, element, the Jacobi detector and rebuild): the dispatch records its
+  // own start and end; the element-order sweep (several launches) is bracketed by stream events
+  auto single = [lim](int part) { return part <= Solver::PART_ELEMENT || lim == DGM_LIMITER_JACOBI; };
   double acc[NP_] = {0, 0, 0, 0, 0};
   double t_dt = 0.0, t_ctl = 0.0, t_step = 0.0;
   for (int it = 0; it < n; ++it) {
-    hipEvent_t* ctl_ev = &ev[2 * 3 * NP_];
     HIPCHK(c, hipEventRecord(ctl_ev[0], c->stream));
     if (!fused_dt(c)) solver(c).time_step(S, c->stream, 0);
     HIPCHK(c, hipEventRecord(ctl_ev[1], c->stream));
@@ -552,11 +566,9 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
     HIPCHK(c, hipEventRecord(ctl_ev[2], c->stream));
     for (int st = 0; st < 3; ++st)
       for (int part = 0; part < NP_; ++part) {
+        if (!launches(part)) continue;
         hipEvent_t* pe = &ev[2 * (st * NP_ + part)];
-        // one launch per part (surface, element, and every part of the Jacobi limiter): the dispatch
-        // records its own start and end; otherwise stream events bracket the part
-        const bool single = part <= Solver::PART_ELEMENT || lim == DGM_LIMITER_JACOBI;
-        if (single) {
+        if (single(part)) {
           S.prof_ev[0] = pe[0];
           S.prof_ev[1] = pe[1];
           solver(c).stage_part(S, st, part, c->stream);
@@ -567,20 +579,24 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
           HIPCHK(c, hipEventRecord(pe[1], c->stream));
         }
       }
+    HIPCHK(c, hipEventRecord(ctl_ev[3], c->stream));
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[0], ctl_ev[1]));
-    t_dt += ms;
+    if (!fused_dt(c)) {  // with the Jacobi limiter the reduction is part of the last detector pass
+      HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[0], ctl_ev[1]));
+      t_dt += ms;
+    }
     HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[1], ctl_ev[2]));
     t_ctl += ms;
     for (int st = 0; st < 3; ++st)
       for (int part = 0; part < NP_; ++part) {
+        if (!launches(part)) continue;
         hipEvent_t* pe = &ev[2 * (st * NP_ + part)];
         HIPCHK(c, hipEventElapsedTime(&ms, pe[0], pe[1]));
         acc[part] += ms;
       }
-    HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[0], ev[2 * (2 * NP_ + NP_ - 1) + 1]));
+    HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[0], ctl_ev[3]));
     t_step += ms;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
@@ -609,6 +625,7 @@ int dgm_troubled_counts(dgm_ctx* c, int32_t counts[4]) {
 int dgm_lift_coefficient(dgm_ctx* c, double chord, double* cl) {
   if (!c || !c->stream || !cl || !(chord > 0.0)) return DGM_BAD_ARGUMENT;
   HIPCHK(c, hipMemsetAsync(c->d_force, 0, 2 * sizeof(double), c->stream));
+  solver(c).flush(dev(c), c->stream);
   solver(c).wall_force(dev(c), c->d_force, 0, (int)c->M.nedge, c->stream);
   double f[2];
   HIPCHK(c, hipMemcpyAsync(f, c->d_force, sizeof(f), hipMemcpyDeviceToHost, c->stream));

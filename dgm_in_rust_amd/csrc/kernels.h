@@ -84,12 +84,17 @@ struct StageBufs {
   T* s;            // FP32-bulk: float stage state
   T* d;            // FP32-bulk: stage increment u^(s) - u^n
   double* rout;    // OP_RESID output, [4 Np][nes]
+  // Deferred limiter output (Jacobi march): pend[e] >= 0 means element e's stage input is the rebuilt
+  // polynomial pw[pend[e]][4][Np] of the last limiter pass, not the stored state; nullptr: none.
+  const int* pend;
+  const double* pw;
 };
 
 struct LimiterBufs {
   int* list;       // troubled element ids
   double* w;       // reconstructed polynomials [slot][4][Np]
   uint8_t* flags;  // optional per-element troubled flags
+  int* pend;       // per element: its slot in w after the last detector pass, or -1
 };
 
 // Launch shapes of one arena, chosen by the autotuner (defaults are the measured best at p = 3).

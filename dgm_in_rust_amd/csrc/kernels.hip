@@ -250,6 +250,43 @@ __device__ __forceinline__ int case_of(const DevMesh& m, const int* __restrict__
   return map[__builtin_amdgcn_readfirstlane(index >> 6)];
 }
 
+// store a rebuilt element (double [4][NP]) into the buffers of the given stage
+template <class T, int NP, int STAGE>
+__device__ __forceinline__ void store_limited(const StageBufs<T>& b, int nes, int e, int idx, double val) {
+  const int v = idx / NP, k = idx % NP;
+  const size_t ix = (size_t)(v * NP + k) * nes + e;
+  if constexpr (sizeof(T) == 8) {
+    if (STAGE == 2)
+      b.un[ix] = val;
+    else
+      b.us[ix] = (T)val;
+  } else {
+    if (STAGE == 2) {
+      b.un[ix] = val;
+      b.sn[ix] = (T)val;
+    } else {
+      b.s[ix] = (T)val;
+      b.d[ix] = (T)(val - b.un[ix]);
+    }
+  }
+}
+
+// A case that has stopped no longer reads its pending rebuilds, and later limiter passes of other cases
+// reuse their slots: the first stage-0 element pass after it stops writes them back (u^n and the FP32
+// shadow) and clears the marks, before this step's limiter reuses the slots.
+template <class T, int NP>
+__device__ __forceinline__ void write_back_pending(const StageBufs<T>& b, int e, int nes) {
+  const int sl = b.pend[e];
+  if (sl < 0) return;
+  const double* w = b.pw + (size_t)sl * 4 * NP;
+#pragma unroll 1
+  for (int c = 0; c < 4 * NP; ++c) {
+    b.un[(size_t)c * nes + e] = w[c];
+    if constexpr (sizeof(T) == 4) b.sn[(size_t)c * nes + e] = (float)w[c];
+  }
+  const_cast<int*>(b.pend)[e] = -1;
+}
+
 // Appends the lanes with pred set to a list counter with one atomic per wavefront; returns each such
 // lane's slot (-1 for the others). Every active lane of the wavefront must call it.
 __device__ __forceinline__ int wave_append(int* counter, bool pred, int* also) {
@@ -275,9 +312,28 @@ __device__ __forceinline__ unsigned long long* dt_slot(const Ctl& ctl, int c, in
 // ---------------------------------------------------------------------------------------
 // surface kernel: one lane per edge
 // ---------------------------------------------------------------------------------------
+
+// traces tr[q][v] at the NE points of local edge lic (point order reversed with rev) of the modes
+// coef(v, k)
+template <class T, int NP, int NE, class F>
+__device__ __forceinline__ void edge_traces(F coef, const T* sphi, int lic, bool rev, T tr[NE][4]) {
+#pragma unroll
+  for (int q = 0; q < NE; ++q)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tr[q][v] = T(0);
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const T c = coef(v, k);
+#pragma unroll
+      for (int q = 0; q < NE; ++q) tr[q][v] += c * sphi[(lic * NE + (rev ? NE - 1 - q : q)) * NP + k];
+    }
+}
 template <class T, int NP, int NQ, int NE>
 __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict__ U, T* __restrict__ G,
-                                                 const T* __restrict__ tab, Ctl ctl, int force, int remap) {
+                                                 const T* __restrict__ tab, Ctl ctl, int force, int remap,
+                                                 const int* __restrict__ pend, const double* __restrict__ pw) {
   using O = TabOff<NP, NQ, NE>;
   __shared__ T sphi[3 * NE * NP];
   for (int i = threadIdx.x; i < 3 * NE * NP; i += blockDim.x) sphi[i] = tab[O::PHIE + i];
@@ -297,35 +353,22 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
   const T* eg = geo_ed<T>(m);
   const T nx = eg[ed], ny = eg[est + ed], hl = eg[2 * est + ed];
   const T hcr = (T)m.hcr;
+  // an element rebuilt by the last limiter pass and not yet written back is read from the rebuild
+  // (rare: the stored modes are loaded regardless, so the common path waits on no extra load)
+  const int pl = pend ? pend[L] : -1;
   T ul[NE][4];
-#pragma unroll
-  for (int q = 0; q < NE; ++q)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) ul[q][v] = T(0);
-#pragma unroll
-  for (int v = 0; v < 4; ++v)
-#pragma unroll
-    for (int k = 0; k < NP; ++k) {
-      const T c = U[(size_t)(v * NP + k) * nes + L];
-#pragma unroll
-      for (int q = 0; q < NE; ++q) ul[q][v] += c * sphi[(licL * NE + q) * NP + k];
-    }
+  edge_traces<T, NP, NE>([&](int v, int k) { return U[(size_t)(v * NP + k) * nes + L]; }, sphi, licL, false, ul);
+  if (pl >= 0)
+    edge_traces<T, NP, NE>([&](int v, int k) { return (T)pw[(size_t)pl * 4 * NP + v * NP + k]; }, sphi, licL, false, ul);
   T* g = G + (size_t)ed * NE * 4;
   if (bc == BC_INTERIOR) {
     const int R = m.ed_elem[est + ed];
+    const int pr = pend ? pend[R] : -1;
     T ur[NE][4];
-#pragma unroll
-    for (int q = 0; q < NE; ++q)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) ur[q][v] = T(0);
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        const T c = U[(size_t)(v * NP + k) * nes + R];
-#pragma unroll
-        for (int q = 0; q < NE; ++q) ur[q][v] += c * sphi[(licR * NE + (NE - 1 - q)) * NP + k];
-      }
+    edge_traces<T, NP, NE>([&](int v, int k) { return U[(size_t)(v * NP + k) * nes + R]; }, sphi, licR, true, ur);
+    if (pr >= 0)
+      edge_traces<T, NP, NE>([&](int v, int k) { return (T)pw[(size_t)pr * 4 * NP + v * NP + k]; }, sphi, licR, true,
+                             ur);
     int err = 0;
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
@@ -372,7 +415,10 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   const int cs = case_of(m, m.el_case64, e);
   if (cs < 0) return;
   const CaseCtl& cc = ctl.cases[cs];
-  if (!force && !case_runs(cc)) return;
+  if (!force && !case_runs(cc)) {
+    if (OP == OP_S0 && b.pend && cc.live && e < cc.e_end) write_back_pending<T, NP>(b, e, m.nes);
+    return;
+  }
   if (e >= cc.e_end) return;
   const int nes = m.nes;
   T u[4][NP];
@@ -380,6 +426,21 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   for (int v = 0; v < 4; ++v)
 #pragma unroll
     for (int k = 0; k < NP; ++k) u[v][k] = b.in[(size_t)(v * NP + k) * nes + e];
+  // a rebuild still pending from the limiter pass before: use it, and write it back (rare)
+  if (OP != OP_RESID && b.pend) {
+    const int sl = b.pend[e];
+    if (sl >= 0) {
+      const double* w = b.pw + (size_t)sl * 4 * NP;
+      constexpr int PREV = OP == OP_S0 ? 2 : (OP == OP_S1 ? 0 : 1);
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          u[v][k] = (T)w[v * NP + k];
+          store_limited<T, NP, PREV>(b, nes, e, v * NP + k, w[v * NP + k]);
+        }
+    }
+  }
   const T* eg = geo_el<T>(m);
   const T xx = eg[G_XX * nes + e], xy = eg[G_XY * nes + e], ex = eg[G_EX * nes + e], ey = eg[G_EY * nes + e];
   const T j2 = eg[G_2J * nes + e];
@@ -517,7 +578,12 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
   const int cs = case_of(m, m.el_case64, wave * 16);
   if (cs < 0) return;
   const CaseCtl& cc = ctl.cases[cs];
-  if (!force && !case_runs(cc)) return;
+  if (!force && !case_runs(cc)) {
+    const int ew = wave * 16 + (lane & 15);
+    if (OP == OP_S0 && b.pend && cc.live && lane < 16 && ew < m.ne && ew < cc.e_end)
+      write_back_pending<float, NP>(b, ew, m.nes);
+    return;
+  }
   const int c = lane & 15, g = lane >> 4;
   const int e = wave * 16 + c;
   const bool valid = e < m.ne && e < cc.e_end;
@@ -544,6 +610,25 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
       const int k = 4 * g + s;
       us[v][s] = (k < NP) ? b.in[(size_t)(v * NP + k) * nes + es] : 0.f;
     }
+  // a rebuild still pending from the limiter pass before: use it and write it back, each lane the
+  // modes it holds (rare)
+  if (OP != OP_RESID && b.pend && valid) {
+    const int sl = b.pend[e];
+    if (sl >= 0) {
+      const double* w = b.pw + (size_t)sl * 4 * NP;
+      constexpr int PREV = OP == OP_S0 ? 2 : (OP == OP_S1 ? 0 : 1);
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int k = 4 * g + s;
+          if (k < NP) {
+            us[v][s] = (float)w[v * NP + k];
+            store_limited<float, NP, PREV>(b, nes, e, v * NP + k, w[v * NP + k]);
+          }
+        }
+    }
+  }
   // values at the cell points q = 4g + r of element e
   f32x4 uq[4];
 #pragma unroll
@@ -856,6 +941,7 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
     }
     const int i = wave_append(&ctl.count[slot], t, &cc->count[slot]);
     if (i >= 0) lb.list[i] = e;
+    if (valid && lb.pend) lb.pend[e] = i;
     return;
   }
   bool t = false;
@@ -872,6 +958,7 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
   }
   const int i = wave_append(&ctl.count[slot], t, &cc->count[slot]);
   if (i >= 0) lb.list[i] = e;
+  if (valid && lb.pend) lb.pend[e] = i;
   wave_min_dt(dt, dt_slot(ctl, cs, cc->step + 1), e >> 6);
 }
 
@@ -1097,26 +1184,6 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
   return 0;
 }
 
-// store a rebuilt element (double [4][NP]) into the buffers of the given stage
-template <class T, int NP, int STAGE>
-__device__ __forceinline__ void store_limited(const StageBufs<T>& b, int nes, int e, int idx, double val) {
-  const int v = idx / NP, k = idx % NP;
-  const size_t ix = (size_t)(v * NP + k) * nes + e;
-  if constexpr (sizeof(T) == 8) {
-    if (STAGE == 2)
-      b.un[ix] = val;
-    else
-      b.us[ix] = (T)val;
-  } else {
-    if (STAGE == 2) {
-      b.un[ix] = val;
-      b.sn[ix] = (T)val;
-    } else {
-      b.s[ix] = (T)val;
-      b.d[ix] = (T)(val - b.un[ix]);
-    }
-  }
-}
 
 // The troubled list holds elements of running cases only (the detector skips the others).
 template <class T, int P, int NQ, int NE>
@@ -1150,7 +1217,25 @@ __global__ __launch_bounds__(256) void k_apply(LimiterBufs lb, StageBufs<T> b, i
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n * 4 * NP; t += gridDim.x * blockDim.x) {
     const int s = t / (4 * NP), idx = t % (4 * NP);
     store_limited<T, NP, STAGE>(b, nes, lb.list[s], idx, lb.w[(size_t)s * 4 * NP + idx]);
+    if (idx == 0 && lb.pend) lb.pend[lb.list[s]] = -1;
   }
+}
+
+// writes back the rebuilt elements still pending after a last-stage limiter pass (u^n and its FP32
+// shadow) and clears their marks
+template <class T, int NP>
+__global__ void k_flush(int* __restrict__ pend, const double* __restrict__ w, double* __restrict__ un,
+                        float* __restrict__ sn, int ne, int nes) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int sl = pend[e];
+  if (sl < 0) return;
+  for (int c = 0; c < 4 * NP; ++c) {
+    const double v = w[(size_t)sl * 4 * NP + c];
+    un[(size_t)c * nes + e] = v;
+    if (sn) sn[(size_t)c * nes + e] = (float)v;
+  }
+  pend[e] = -1;
 }
 
 // exact element-order sweep: one wavefront per element of one level; elements of a level are not
@@ -1311,6 +1396,9 @@ class SolverImpl final : public Solver {
     b.s = sizeof(T) == 4 ? (T*)s.s : nullptr;
     b.d = sizeof(T) == 4 ? (T*)s.d : nullptr;
     b.rout = s.rout;
+    // the Jacobi sweep of a march leaves its rebuilds pending for the next stage's kernels to read
+    b.pend = s.limiter == DGM_LIMITER_JACOBI ? s.lim_pend : nullptr;
+    b.pw = s.lim_w;
     return b;
   }
   // stage input: FP64 reads u^n then the stage state; FP32-bulk reads the float shadows
@@ -1323,11 +1411,11 @@ class SolverImpl final : public Solver {
     return stage == 2 ? (const T*)s.sn : (const T*)s.s;
   }
 
-  void surface(const DeviceState& s, const T* in, hipStream_t st, int force) const {
+  void surface(const DeviceState& s, const StageBufs<T>& b, hipStream_t st, int force) const {
     if (s.mesh.nedge <= 0) return;
     const int bs = s.tune.surface_block;
-    go(s, k_surface<T, NP, NQ, NE>, blocks_for(s.mesh.nedge, bs), bs, st, s.mesh, in, (T*)s.flux, (const T*)s.tab,
-       s.ctl, force, s.tune.remap);
+    go(s, k_surface<T, NP, NQ, NE>, blocks_for(s.mesh.nedge, bs), bs, st, s.mesh, b.in, (T*)s.flux, (const T*)s.tab,
+       s.ctl, force, s.tune.remap, b.pend, b.pw);
   }
   template <int OP>
   void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force) const {
@@ -1348,7 +1436,8 @@ class SolverImpl final : public Solver {
   void residual(const DeviceState& s, hipStream_t st) const override {
     StageBufs<T> b = bufs(s);
     b.in = stage_input(s, 0);
-    surface(s, b.in, st, 1);
+    b.pend = nullptr;  // a stand-alone residual reads the uploaded state as it is
+    surface(s, b, st, 1);
     element<OP_RESID>(s, b, st, 1);
   }
 
@@ -1367,7 +1456,7 @@ class SolverImpl final : public Solver {
     StageBufs<T> b = bufs(s);
     b.in = stage_input(s, stage_idx);
     if (part == PART_SURFACE) {
-      surface(s, b.in, st, 0);
+      surface(s, b, st, 0);
     } else if (part == PART_ELEMENT) {
       if (stage_idx == 0)
         element<OP_S0>(s, b, st, 0);
@@ -1390,7 +1479,7 @@ class SolverImpl final : public Solver {
     if (s.mesh.ne <= 0) return;
     StageBufs<T> b = bufs(s);
     const T* U = limiter_input(s, STAGE);
-    LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags};
+    LimiterBufs lb{s.lim_list, s.lim_w, s.lim_flags, s.lim_pend};
     if (s.limiter == DGM_LIMITER_JACOBI) {
       // the last stage of a step also reduces the next step's time step (not the stand-alone call)
       const int fuse = (STAGE == 2 && slot == 2) ? 1 : 0;
@@ -1400,7 +1489,9 @@ class SolverImpl final : public Solver {
            fuse, s.tune.remap);
       } else if (part == PART_REBUILD) {
         go(s, k_weno<T, P, NQ, NE>, s.tune.weno_grid, 64, st, s.mesh, U, s.smat, lb, s.ctl, slot, fuse);
-      } else if (part == PART_APPLY) {
+      } else if (part == PART_APPLY && slot == 3) {
+        // only the stand-alone limiter writes back at once; in a march the rebuilds stay pending for
+        // the next stage's surface and element kernels (and stage 0 writes back u^n)
         go(s, k_apply<T, NP, STAGE>, 1024, 256, st, lb, b, s.mesh.nes, s.ctl, slot);
       }
     } else if (s.limiter == DGM_LIMITER_SEQUENTIAL && part == PART_DETECT) {
@@ -1439,6 +1530,11 @@ class SolverImpl final : public Solver {
   }
   void check_finite(const DeviceState& s, int* flag, int e0, int n, hipStream_t st) const override {
     if (n > 0) k_check_finite<<<blocks_for(n, 256), 256, 0, st>>>(s.un, e0, n, s.mesh.nes, 4 * NP, flag);
+  }
+  void flush(const DeviceState& s, hipStream_t st) const override {
+    if (s.mesh.ne <= 0 || s.limiter != DGM_LIMITER_JACOBI) return;
+    k_flush<T, NP><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.lim_pend, s.lim_w, s.un, (float*)s.sn, s.mesh.ne,
+                                                              s.mesh.nes);
   }
   void wall_force(const DeviceState& s, double* force, int ed0, int n, hipStream_t st) const override {
     if (n > 0)

@@ -27,6 +27,7 @@ struct DeviceState {
   int* lim_list = nullptr;
   double* lim_w = nullptr;
   uint8_t* lim_flags = nullptr;
+  int* lim_pend = nullptr;     // [nes] slot in lim_w of a rebuilt element not yet written back, or -1
   Ctl ctl{};               // case records (shared by the arenas of a batch) and this arena's list counters
   int limiter = 0;
   int use_mfma = 1;        // FP32-bulk element kernel on the matrix cores (p <= 3)
@@ -66,6 +67,9 @@ class Solver {
   virtual void check_finite(const DeviceState& s, int* flag, int e0, int n, hipStream_t st) const = 0;
   // wall force per case into force[2 case + i], edges ed0 .. ed0 + n - 1 (ed0 a multiple of 64)
   virtual void wall_force(const DeviceState& s, double* force, int ed0, int n, hipStream_t st) const = 0;
+  // writes back every rebuilt element still pending from the last limiter pass (a last stage's pass:
+  // u^n and its FP32 shadow) and clears the marks; needed before the state is read or snapshot
+  virtual void flush(const DeviceState& s, hipStream_t st) const = 0;
 };
 
 std::unique_ptr<Solver> make_solver(int precision, int p, int nq, int ne);
