@@ -766,12 +766,6 @@ __device__ __forceinline__ void wave_min_dt(double dt, unsigned long long* slot_
     atomicMin(&slot_base[(wave_index % DT_LANES) * DT_STRIDE], (unsigned long long)__double_as_longlong(dt));
 }
 
-__device__ __forceinline__ double read_dt_slot(const unsigned long long* slot_base) {
-  double m = 1.0e10;
-  for (int i = 0; i < DT_LANES; ++i) m = fmin(m, __longlong_as_double((long long)slot_base[i * DT_STRIDE]));
-  return m;
-}
-
 // stand-alone time-step reduction into the slot of the step about to begin
 // (one wavefront = one 64-element block = one case; every lane of it reaches the reduction)
 template <class T, int NP>
@@ -794,21 +788,28 @@ __global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, 
 
 // Step control (src/temporal_disc.rs:22-34): count the step that just ran, take its successor's dt
 // (clamped to final_time), and decide whether it runs (step < final_step && t < final_time).
-// One thread per case slot.
-__global__ void k_step_begin(Ctl ctl, int ncases) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per case slot: lanes 0 .. DT_LANES - 1 read and reset the case's dt words in parallel.
+__global__ __launch_bounds__(64) void k_step_begin(Ctl ctl, int ncases) {
+  const int i = blockIdx.x;
   if (i >= ncases) return;
   CaseCtl& c = ctl.cases[i];
   if (!c.live) return;
+  const int lane = threadIdx.x;
+  const int step = c.step + (c.pending ? 1 : 0);  // the step about to begin
+  unsigned long long* slot = dt_slot(ctl, i, step);
+  double v = 1.0e10;
+  if (lane < DT_LANES) {
+    v = __longlong_as_double((long long)slot[lane * DT_STRIDE]);
+    slot[lane * DT_STRIDE] = (unsigned long long)__double_as_longlong(1.0e10);
+  }
+  const double dtm = wave_min(v);
+  if (lane != 0) return;
   if (c.pending) {
     c.t += c.dt;
     c.step += 1;
     c.pending = 0;
     c.last_count = c.count[2];
   }
-  unsigned long long* slot = dt_slot(ctl, i, c.step);
-  const double dtm = read_dt_slot(slot);
-  for (int k = 0; k < DT_LANES; ++k) slot[k * DT_STRIDE] = (unsigned long long)__double_as_longlong(1.0e10);
   const bool go = c.step < c.final_step && c.t < c.final_time && c.error == ~0ull;
   c.active = go ? 1 : 0;
   if (go) {
@@ -1369,7 +1370,7 @@ void go(const DeviceState& s, void (*kernel)(KArgs...), dim3 grid, dim3 block, h
 }  // namespace
 
 void launch_step_begin(Ctl ctl, int ncases, hipStream_t st) {
-  if (ncases > 0) k_step_begin<<<blocks_for(ncases, 64), 64, 0, st>>>(ctl, ncases);
+  if (ncases > 0) k_step_begin<<<ncases, 64, 0, st>>>(ctl, ncases);
 }
 
 // ---------------------------------------------------------------------------------------
