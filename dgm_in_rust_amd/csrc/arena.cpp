@@ -200,6 +200,7 @@ int Arena::dalloc(X** p, size_t n, std::string& err) {
 
 Arena::~Arena() {
   if (stream) (void)hipStreamSynchronize(stream);
+  if (hbuf_) (void)hipFree(hbuf_);
   for (void* p : allocs_) (void)hipFree(p);
   if (pinned_) (void)hipHostFree(pinned_);
 }
@@ -443,7 +444,7 @@ int Arena::place(const HostMesh& M, const std::vector<double>& patch_q, int case
     el_geo[G_XY * ne + e] = M.el_metric[4 * e + 1];
     el_geo[G_EX * ne + e] = M.el_metric[4 * e + 2];
     el_geo[G_EY * ne + e] = M.el_metric[4 * e + 3];
-    el_geo[G_CR * ne + e] = M.el_cr[e];
+    el_geo[G_CR15 * ne + e] = std::pow(M.el_cr[e], 1.5);
     el_geo[G_HMIN * ne + e] = M.el_hmin[e];
     el_geo[G_2J * ne + e] = 2.0 / J;
   }
@@ -512,12 +513,30 @@ int Arena::unplace(const Placement& pl, std::string& err) {
   return DGM_OK;
 }
 
+int Arena::grow_staging(size_t n, std::string& err) {
+  if (n <= hbuf_size_) return DGM_OK;
+  ACHK(hipStreamSynchronize(stream));
+  if (hbuf_) {
+    ACHK(hipFree(hbuf_));
+    bytes -= (int64_t)(hbuf_size_ * sizeof(double));
+  }
+  hbuf_ = nullptr;
+  hbuf_size_ = 0;
+  void* v = nullptr;
+  const hipError_t e = hipMalloc(&v, n * sizeof(double));
+  if (e != hipSuccess) {
+    err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return DGM_OUT_OF_MEMORY;
+  }
+  hbuf_ = (double*)v;
+  hbuf_size_ = n;
+  bytes += (int64_t)(n * sizeof(double));
+  return DGM_OK;
+}
+
 int Arena::upload_state(const double* U, const Placement& pl, std::string& err) {
   const size_t n = (size_t)pl.ne * 4 * T.np;
-  if (n > hbuf_size_) {
-    if (int s = dalloc(&hbuf_, n, err)) return s;
-    hbuf_size_ = n;
-  }
+  if (int s = grow_staging(n, err)) return s;
   ACHK(hipMemcpyAsync(hbuf_, U, n * 8, hipMemcpyHostToDevice, stream));
   solver->upload(S, hbuf_, (int)pl.e_off, (int)pl.ne, stream);
   // nothing of an earlier state stays pending over the uploaded one
@@ -529,10 +548,7 @@ int Arena::upload_state(const double* U, const Placement& pl, std::string& err) 
 
 int Arena::download_state(const double* soa, double* U, const Placement& pl, std::string& err) {
   const size_t n = (size_t)pl.ne * 4 * T.np;
-  if (n > hbuf_size_) {
-    if (int s = dalloc(&hbuf_, n, err)) return s;
-    hbuf_size_ = n;
-  }
+  if (int s = grow_staging(n, err)) return s;
   solver->flush(S, stream);
   solver->download(S, soa, hbuf_, (int)pl.e_off, (int)pl.ne, stream);
   ACHK(hipGetLastError());

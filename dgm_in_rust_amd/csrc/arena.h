@@ -90,6 +90,7 @@ class Arena {
   int dalloc(X** p, size_t n, std::string& err);
   int stage_copy(const std::vector<std::pair<void*, std::pair<const void*, size_t>>>& pieces, std::string& err);
   int set_case64(int* map, int64_t first, int64_t n, int value, std::string& err);
+  int grow_staging(size_t n, std::string& err);  // device buffer of n doubles for host-layout states
   std::vector<void*> allocs_;
   void* pinned_ = nullptr;
   size_t pinned_size_ = 0;

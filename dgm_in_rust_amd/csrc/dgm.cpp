@@ -79,13 +79,10 @@ int read_ctl(dgm_ctx* c, CaseCtl& h) {
 // writes the control record and empties both dt slots
 int write_ctl(dgm_ctx* c, const CaseCtl& h) {
   if (!c->stream) return fail(c, DGM_BAD_ARGUMENT, "host-only context");
-  static std::vector<unsigned long long> empty;
-  if (empty.empty()) {
-    const double big = 1.0e10;
-    unsigned long long b;
-    std::memcpy(&b, &big, 8);
-    empty.assign(DT_CASE, b);
-  }
+  const double big = 1.0e10;
+  unsigned long long b;
+  std::memcpy(&b, &big, 8);
+  const std::vector<unsigned long long> empty(DT_CASE, b);
   HIPCHK(c, hipMemcpyAsync(c->d_case, &h, sizeof(CaseCtl), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_dtmin, empty.data(), DT_CASE * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

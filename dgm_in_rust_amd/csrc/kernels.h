@@ -19,7 +19,9 @@
 
 namespace dgm {
 
-enum ElemGeo { G_J = 0, G_XX = 1, G_XY = 2, G_EX = 3, G_EY = 4, G_CR = 5, G_HMIN = 6, G_2J = 7, G_N = 8 };
+// element geometry rows: J, metric (dxi/dx, dxi/dy, deta/dx, deta/dy), circumradius^1.5 (the KXRCF
+// scale), minimum height, 2/J
+enum ElemGeo { G_J = 0, G_XX = 1, G_XY = 2, G_EX = 3, G_EY = 4, G_CR15 = 5, G_HMIN = 6, G_2J = 7, G_N = 8 };
 enum EdgeBc { BC_INTERIOR = 0, BC_WALL = 1, BC_FARFIELD = 2 };
 // element kernel operations: residual only, or residual fused with RK stage 1, 2, 3
 enum ElemOp { OP_RESID = 0, OP_S0 = 1, OP_S1 = 2, OP_S2 = 3 };

@@ -829,7 +829,9 @@ __global__ void k_reset_count(Ctl ctl, int slot) { ctl.count[slot] = 0; }
 
 // modified_kxrcf, src/spatial_disc/limiter/weno.rs:7-85. The neighbour trace uses the target's own
 // local edge index with reversed point order (:21-27, :63-64); faces without a neighbour are skipped;
-// no inflow face gives 0/0 = NaN, which is not >= 1.
+// no inflow face gives 0/0 = NaN, which is not >= 1. The detection arithmetic runs in the bulk type
+// (FP64 in FP64 mode; FP32 in FP32-bulk mode, where only the WENO indicators and weights and the
+// positivity tests stay in FP64); the mode sums for the next time step are FP64 in both.
 template <class T, int P, int NQ, int NE>
 __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, int e,
                                       double& sum0, double& sum1, double& sum2, double& sum3) {
@@ -859,7 +861,8 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
     sum2 = a2;
     sum3 = a3;
   }
-  double djump = 0.0, ejump = 0.0, surface = 0.0, minr = 1.0e10, mine = 1.0e10;
+  using A = T;  // detection arithmetic in the bulk type
+  A djump = 0, ejump = 0, surface = 0, minr = A(1.0e10), mine = A(1.0e10);
   const double* phie = lt + LO::PHIE;
   const double* ew = lt + LO::EW;
 #pragma unroll
@@ -868,54 +871,54 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
     if (nb < 0) continue;
     const int es = m.el_edge[lic * nes + e];
     const int edge = es >> 1;
-    const double sgn = (es & 1) ? -1.0 : 1.0;
-    const double nx = sgn * m.ed_geo[edge], ny = sgn * m.ed_geo[m.ed_stride + edge];
-    const double len = 2.0 * m.ed_geo[2 * m.ed_stride + edge];
-    double tr[NE], te[NE];
+    const A sgn = (es & 1) ? A(-1) : A(1);
+    const A nx = sgn * (A)m.ed_geo[edge], ny = sgn * (A)m.ed_geo[m.ed_stride + edge];
+    const A len = A(2) * (A)m.ed_geo[2 * m.ed_stride + edge];
+    A tr[NE], te[NE];
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
-      double a = 0.0, b = 0.0;
+      A a = 0, b = 0;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        const double ph = phie[(lic * NE + q) * NP + k];
-        a += (double)rho[k] * ph;
-        b += (double)en[k] * ph;
+        const A ph = (A)phie[(lic * NE + q) * NP + k];
+        a += (A)rho[k] * ph;
+        b += (A)en[k] * ph;
       }
       tr[q] = a;
       te[q] = b;
       if (a < minr) minr = a;
       if (b < mine) mine = b;
     }
-    double mu = 0.0, mv = 0.0;
+    A mu = 0, mv = 0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      const double ph = phie[(lic * NE + NE / 2) * NP + k];
-      mu += (double)ru[k] * ph;
-      mv += (double)rv[k] * ph;
+      const A ph = (A)phie[(lic * NE + NE / 2) * NP + k];
+      mu += (A)ru[k] * ph;
+      mv += (A)rv[k] * ph;
     }
-    if (mu * nx + mv * ny < 0.0) {
-      double rj = 0.0, ej = 0.0;
+    if (mu * nx + mv * ny < A(0)) {
+      A rj = 0, ej = 0;
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
-        double xr = 0.0, xe = 0.0;
+        A xr = 0, xe = 0;
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
-          const double ph = phie[(lic * NE + (NE - 1 - q)) * NP + k];
-          xr += (double)U[(size_t)(0 * NP + k) * nes + nb] * ph;
-          xe += (double)U[(size_t)(3 * NP + k) * nes + nb] * ph;
+          const A ph = (A)phie[(lic * NE + (NE - 1 - q)) * NP + k];
+          xr += (A)U[(size_t)(0 * NP + k) * nes + nb] * ph;
+          xe += (A)U[(size_t)(3 * NP + k) * nes + nb] * ph;
         }
-        rj += 0.5 * len * ew[q] * (tr[q] - xr);
-        ej += 0.5 * len * ew[q] * (te[q] - xe);
+        rj += A(0.5) * len * (A)ew[q] * (tr[q] - xr);
+        ej += A(0.5) * len * (A)ew[q] * (te[q] - xe);
       }
       djump += rj;
       ejump += ej;
       surface += len;
     }
   }
-  const double h15 = pow(m.el_geo[G_CR * nes + e], 1.5);
-  const double id = fabs(djump) / (h15 * surface * minr);
-  const double ie = fabs(ejump) / (h15 * surface * mine);
-  return (id >= 1.0) || (ie >= 1.0);
+  const A h15 = (A)m.el_geo[G_CR15 * nes + e];
+  const A id = fabs(djump) / (h15 * surface * minr);
+  const A ie = fabs(ejump) / (h15 * surface * mine);
+  return (id >= A(1)) || (ie >= A(1));
 }
 
 // KXRCF detector, one lane per element. With fuse_dt (the last stage of a step) the next step's time
