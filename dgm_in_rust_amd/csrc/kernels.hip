@@ -1059,6 +1059,7 @@ struct WenoShared {
   double PN[3][4][NP];
   double RES[3][4][NP];
   double area[3];
+  double fin[4 * NP];  // the rebuilt polynomial, for the time step of the fused reduction
   int nb[3];
   int edge[3];
   int flag;
@@ -1071,26 +1072,40 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
   constexpr int NPK = NP * (NP + 1) / 2;
   const int lane = threadIdx.x;
   const int nes = m.nes;
+  // neighbour and edge ids of the three faces, read once and broadcast over the wavefront; every
+  // other load depends on them only, so all are issued together before one barrier
+  int nbl = -1, edl = 0;
   if (lane < 3) {
-    const int nb = m.el_nbr[lane * nes + e];
-    sh.nb[lane] = nb;
-    sh.edge[lane] = m.el_edge[lane * nes + e] >> 1;
-    sh.area[lane] = nb >= 0 ? m.el_geo[G_J * nes + nb] * 0.5 : 0.0;
+    nbl = m.el_nbr[lane * nes + e];
+    edl = m.el_edge[lane * nes + e] >> 1;
+  }
+  const int nbr[3] = {__shfl(nbl, 0, 64), __shfl(nbl, 1, 64), __shfl(nbl, 2, 64)};
+  const int edr[3] = {__shfl(edl, 0, 64), __shfl(edl, 1, 64), __shfl(edl, 2, 64)};
+  if (lane < 3) {
+    sh.nb[lane] = nbl;
+    sh.edge[lane] = edl;
+    sh.area[lane] = nbl >= 0 ? m.el_geo[G_J * nes + nbl] * 0.5 : 0.0;
   }
   for (int idx = lane; idx < NPK; idx += 64) sh.B[idx] = smat[(size_t)e * NPK + idx];
-  __syncthreads();
   for (int idx = lane; idx < 16 * NP; idx += 64) {
     const int j = idx / (4 * NP), v = (idx / NP) % 4, k = idx % NP;
+    const int nj = j > 0 ? nbr[j - 1] : e;
     double val = 0.0;
     if (j == 0 || k == 0)
       val = (double)U[(size_t)(v * NP + k) * nes + e];
-    else if (sh.nb[j - 1] >= 0)
-      val = (double)U[(size_t)(v * NP + k) * nes + sh.nb[j - 1]];
-    if (j > 0 && sh.nb[j - 1] < 0) val = 0.0;
+    else if (nj >= 0)
+      val = (double)U[(size_t)(v * NP + k) * nes + nj];
+    if (j > 0 && nj < 0) val = 0.0;  // no neighbour across the face: no candidate
     sh.src[j][v][k] = val;
   }
+  // face normal of the characteristic lanes (lane 16 i + ...: face i)
+  double fnx = 0.0, fny = 0.0;
+  if (lane < 48 && nbr[lane >> 4] >= 0) {
+    fnx = m.ed_geo[edr[lane >> 4]];
+    fny = m.ed_geo[m.ed_stride + edr[lane >> 4]];
+  }
   // linear weights, weno.rs:93-110 (a boundary face i zeroes weights[i], then weights[count] is set)
-  const bool b0 = sh.nb[0] < 0, b1 = sh.nb[1] < 0, b2 = sh.nb[2] < 0;
+  const bool b0 = nbr[0] < 0, b1 = nbr[1] < 0, b2 = nbr[2] < 0;
   const int nbnd = (int)b0 + (int)b1 + (int)b2;
   double lw0 = b0 ? 0.0 : 0.997, lw1 = b1 ? 0.0 : 0.001, lw2 = b2 ? 0.0 : 0.001, lw3 = 0.001;
   if (nbnd == 0) lw0 = 0.997;
@@ -1106,7 +1121,7 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
 #pragma unroll
     for (int k = 0; k < NP; ++k) Pk[k] = 0.0;
     if (sh.nb[i] >= 0) {
-      const double nx = m.ed_geo[sh.edge[i]], ny = m.ed_geo[m.ed_stride + sh.edge[i]];
+      const double nx = fnx, ny = fny;
       const double mean[4] = {sh.src[0][0][0], sh.src[0][1][0], sh.src[0][2][0], sh.src[0][3][0]};
       double Lm[16], Rm[16];
       eigenmatrix(mean, nx, ny, hcr, Lm, Rm);
@@ -1184,6 +1199,7 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
         area += sh.area[i];
       }
     out[idx] = rec / area;
+    sh.fin[idx] = rec / area;
   }
   return 0;
 }
@@ -1205,7 +1221,7 @@ __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U,
     if (st && threadIdx.x == 0) atomicMin(&cc->error, (0xFFFFFFull << 8) | (unsigned long long)st);
     __syncthreads();
     if (fuse_dt && !st && threadIdx.x == 0) {
-      const double d = element_dt<NP>([&](int v, int k) { return out[v * NP + k]; }, m.hcr, cc->cfl,
+      const double d = element_dt<NP>([&](int v, int k) { return sh.fin[v * NP + k]; }, m.hcr, cc->cfl,
                                       m.el_geo[G_HMIN * m.nes + e]);
       if (d < 1.0e10)
         atomicMin(dt_slot(ctl, cs, cc->step + 1) + (e % DT_LANES) * DT_STRIDE,
