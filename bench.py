@@ -22,10 +22,10 @@ Also measured in the same run (sub-objects of the JSON line):
 * sweep: 256 airfoil cases (log-uniform 2k-60k triangles, p in {1, 2, 3}, M in [0.3, 0.85], AoA in
   [-4, 6] deg) advanced together in one batch for 1000 steps each, FP32-bulk, autotuned launch shapes;
   aggregate GDOF * stage / s.
-* serving: Poisson arrivals at 2, 8 and 32 requests/s of cases from the same distribution, each run
-  until its lift coefficient changes by less than 1e-5 over 100 steps or for 20 000 steps; p50 / p99
-  time-to-converged-lift (arrival to exit), throughput and peak concurrency, FP32-bulk at every rate and
-  FP64 at the middle rate.
+* serving: Poisson arrivals at 2, 8 and 32 requests/s (16, 32 and 96 requests) of cases from the
+  same distribution, each run until its lift coefficient changes by less than 1e-5 over 100 steps or
+  for 20 000 steps; p50 / p99 time-to-converged-lift (arrival to exit), throughput and peak
+  concurrency, FP32-bulk at every rate and FP64 at the middle rate.
 """
 import argparse
 import json
@@ -164,7 +164,7 @@ def sweep(ncases, steps, seed=0):
                 precision="fp32_bulk", limiter="jacobi", seed=seed)
 
 
-def serve(rate, nreq, precision, max_steps=20000, cl_tol=1e-5, seed=0, time_limit=60.0):
+def serve(rate, nreq, precision, max_steps=20000, cl_tol=1e-5, seed=0, time_limit=90.0):
     """Serving loop at Poisson rate `rate`/s over nreq cases of the sweep distribution; each runs until
     |dCl| < cl_tol over 100 steps or max_steps. Returns p50/p99 time-to-converged-lift and throughput."""
     from dgm_in_rust_amd import Batch, cases, meshgen
@@ -212,7 +212,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--sweep-cases", type=int, default=256, help="batched sweep size (0: skip)")
     ap.add_argument("--sweep-steps", type=int, default=1000)
-    ap.add_argument("--serve-requests", type=int, default=16, help="requests per serving rate (0: skip)")
+    ap.add_argument("--serve-requests", default="16,32,96",
+                    help="requests at each serving rate, comma-separated like --serve-rates (0: skip serving)")
     ap.add_argument("--serve-rates", default="2,8,32")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (counter scale reference for rocprofv3 --pmc)")
@@ -257,15 +258,17 @@ def main():
         print(f"sweep: {sw['value']:.2f} GDOF*stage/s over {sw['cases']} cases ({sw['elements']} elements), "
               f"{sw['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
     sv = None
-    if args.serve_requests > 0:
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    nreq = [int(x) for x in str(args.serve_requests).split(",") if x]
+    if rates and nreq and max(nreq) > 0:
+        nreq = (nreq * len(rates))[:len(rates)] if len(nreq) < len(rates) else nreq[:len(rates)]
         sv = {"fp32_bulk": {}, "fp64": {}}
-        rates = [float(r) for r in args.serve_rates.split(",") if r]
-        for r in rates:
-            sv["fp32_bulk"][f"{r:g}"] = serve(r, args.serve_requests, "fp32")
+        for r, n in zip(rates, nreq):
+            sv["fp32_bulk"][f"{r:g}"] = serve(r, n, "fp32")
             print(f"serve fp32 rate {r:g}: {sv['fp32_bulk'][f'{r:g}']}", file=sys.stderr, flush=True)
-        mid = rates[len(rates) // 2] if rates else 8.0
-        sv["fp64"][f"{mid:g}"] = serve(mid, args.serve_requests, "fp64")
-        print(f"serve fp64 rate {mid:g}: {sv['fp64'][f'{mid:g}']}", file=sys.stderr, flush=True)
+        mid = len(rates) // 2
+        sv["fp64"][f"{rates[mid]:g}"] = serve(rates[mid], nreq[mid], "fp64")
+        print(f"serve fp64 rate {rates[mid]:g}: {sv['fp64'][f'{rates[mid]:g}']}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args.order, mach, aoa, args.cpu_steps)
