@@ -68,7 +68,7 @@ def stage_bytes_survey(np_, fp32):
     return (80.0 * np_ + 53.0) if fp32 else (117.33 * np_ + 91.0)
 
 
-def run_gpu(mesh, precision, steps, warmup, order, mach, aoa):
+def run_gpu(mesh, precision, steps, warmup, order, mach, aoa, tuning=None):
     from dgm_in_rust_amd import Context
     ctx = Context(mesh, order=order, precision=precision, limiter="jacobi", mach=mach, aoa_deg=aoa)
     U0 = ctx.initial_solution(1.0, aoa, mach, 1.0e5)
@@ -76,7 +76,12 @@ def run_gpu(mesh, precision, steps, warmup, order, mach, aoa):
     if warmup:
         ctx.steps_async(warmup)
         ctx.synchronize()
-    tuning = ctx.autotune(repeats=10)   # launch shapes timed on this state (state and clock unchanged)
+    if tuning is None:
+        tuning = ctx.autotune(repeats=10)   # launch shapes timed on this state (state and clock unchanged)
+    else:
+        if tuning:
+            ctx.set_tuning({**ctx.tuning(), **tuning})
+        tuning = ctx.tuning()
     t0 = time.perf_counter()
     ctx.steps_async(steps)
     ctx.synchronize()
@@ -215,6 +220,8 @@ def main():
     ap.add_argument("--serve-requests", default="16,32,96",
                     help="requests at each serving rate, comma-separated like --serve-rates (0: skip serving)")
     ap.add_argument("--serve-rates", default="2,8,32")
+    ap.add_argument("--tuning", default=None,
+                    help="launch shapes as JSON (e.g. '{\"element_waves\": 1}', '{}' = the defaults) instead of the autotuner")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (counter scale reference for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -234,13 +241,14 @@ def main():
     from dgm_in_rust_amd import meshgen
     mach, aoa = 0.8, 1.25
     mesh = meshgen.naca0012_ogrid(args.surface, args.layers, seed=0)
-    r32 = run_gpu(mesh, "fp32", args.steps, args.warmup, args.order, mach, aoa)
+    tun = json.loads(args.tuning) if args.tuning is not None else None
+    r32 = run_gpu(mesh, "fp32", args.steps, args.warmup, args.order, mach, aoa, tun)
     s32 = summarise(r32, args.steps, True)
     print(f"fp32-bulk: {s32['value']:.2f} GDOF*stage/s, {s32['ms_per_step']:.4f} ms/step, kernels {s32['kernel_ms']}",
           file=sys.stderr, flush=True)
     s64 = None
     if not args.no_fp64:
-        r64 = run_gpu(mesh, "fp64", args.steps, args.warmup, args.order, mach, aoa)
+        r64 = run_gpu(mesh, "fp64", args.steps, args.warmup, args.order, mach, aoa, tun)
         s64 = summarise(r64, args.steps, False)
         print(f"fp64: {s64['value']:.2f} GDOF*stage/s, {s64['ms_per_step']:.4f} ms/step, kernels {s64['kernel_ms']}",
               file=sys.stderr, flush=True)
