@@ -560,6 +560,10 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
 // ---------------------------------------------------------------------------------------
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// x where p holds, else +0: a bit mask rather than a select, so that the compiler does not turn the
+// load that produced x into a branch around it (it does so for selects with a load operand)
+__device__ __forceinline__ float keep_if(bool p, float x) { return __int_as_float(__float_as_int(x) & -(int)p); }
+
 __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
@@ -578,29 +582,13 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
   const int cs = case_of(m, m.el_case64, wave * 16);
   if (cs < 0) return;
   const CaseCtl& cc = ctl.cases[cs];
-  if (!force && !case_runs(cc)) {
-    const int ew = wave * 16 + (lane & 15);
-    if (OP == OP_S0 && b.pend && cc.live && lane < 16 && ew < m.ne && ew < cc.e_end)
-      write_back_pending<float, NP>(b, ew, m.nes);
-    return;
-  }
   const int c = lane & 15, g = lane >> 4;
   const int e = wave * 16 + c;
-  const bool valid = e < m.ne && e < cc.e_end;
   const int nes = m.nes;
-  const int es = valid ? e : 0;
-  // per-lane table operands (A of each product)
-  float aPhi[4], aDxi[4], aDeta[4], aLift[4];
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int k = 4 * g + s;  // value product: K index = mode
-    aPhi[s] = (k < NP && c < NQ) ? tab[O::PHIV + c * NP + k] : 0.f;
-    const int q = 4 * g + s;  // volume product: K index = point, row = mode c
-    aDxi[s] = (q < NQ && c < NP) ? tab[O::DXI + q * NP + c] : 0.f;
-    aDeta[s] = (q < NQ && c < NP) ? tab[O::DETA + q * NP + c] : 0.f;
-    const int mp = 4 * g + s;  // surface product: K index = face * NE + point
-    aLift[s] = (mp < 3 * NE && c < NP) ? tab[O::LIFT + (mp / NE * NE + mp % NE) * NP + c] : 0.f;
-  }
+  // Every load below reads from a clamped in-range index and the unused entries are selected away
+  // afterwards, so the loads carry no branches and are issued together: the state, the table
+  // operands, the edge ids of the lift gather and the metric go out before the case record is back.
+  const int es = e < m.ne ? e : 0;
   // state entries held by this lane: U_v[e][k = 4g + s]
   float us[4][4];
 #pragma unroll
@@ -608,12 +596,47 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int k = 4 * g + s;
-      us[v][s] = (k < NP) ? b.in[(size_t)(v * NP + k) * nes + es] : 0.f;
+      const float x = b.in[(size_t)(v * NP + (k < NP ? k : NP - 1)) * nes + es];
+      us[v][s] = keep_if(k < NP, x);
     }
+  // per-lane table operands (A of each product)
+  float aPhi[4], aDxi[4], aDeta[4], aLift[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = 4 * g + s;  // value product: K index = mode; volume: K index = point; lift: face * NE + point
+    const int kp = k < NP ? k : NP - 1, kq = k < NQ ? k : NQ - 1, km = k < 3 * NE ? k : 3 * NE - 1;
+    const int cq = c < NQ ? c : NQ - 1, cp = c < NP ? c : NP - 1;
+    const float ph = tab[O::PHIV + cq * NP + kp];
+    const float dx = tab[O::DXI + kq * NP + cp], de = tab[O::DETA + kq * NP + cp];
+    const float li = tab[O::LIFT + km * NP + cp];
+    aPhi[s] = keep_if(k < NP && c < NQ, ph);
+    aDxi[s] = keep_if(k < NQ && c < NP, dx);
+    aDeta[s] = keep_if(k < NQ && c < NP, de);
+    aLift[s] = keep_if(k < 3 * NE && c < NP, li);
+  }
+  // edge ids of the lane's (face, point) slots 4g + s of the lift product
+  int eid[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int mp = 4 * g + s;
+    eid[s] = m.el_edge[(mp < 3 * NE ? mp / NE : 0) * nes + es];
+  }
+  const float* eg = m.el_geo32;
+  const float xx = eg[G_XX * nes + es], xy = eg[G_XY * nes + es], ex = eg[G_EX * nes + es], ey = eg[G_EY * nes + es];
+  const float j2 = eg[G_2J * nes + es];
+  const int pend_slot = (OP != OP_RESID && b.pend) ? b.pend[es] : -1;
+  if (!force && !case_runs(cc)) {
+    const int ew = wave * 16 + (lane & 15);
+    if (OP == OP_S0 && b.pend && cc.live && lane < 16 && ew < m.ne && ew < cc.e_end)
+      write_back_pending<float, NP>(b, ew, m.nes);
+    return;
+  }
+  const bool valid = e < m.ne && e < cc.e_end;
+  const double dt = cc.dt;
   // a rebuild still pending from the limiter pass before: use it and write it back, each lane the
   // modes it holds (rare)
   if (OP != OP_RESID && b.pend && valid) {
-    const int sl = b.pend[e];
+    const int sl = pend_slot;
     if (sl >= 0) {
       const double* w = b.pw + (size_t)sl * 4 * NP;
       constexpr int PREV = OP == OP_S0 ? 2 : (OP == OP_S1 ? 0 : 1);
@@ -637,9 +660,6 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
 #pragma unroll
     for (int s = 0; s < 4; ++s) uq[v] = mfma4(aPhi[s], us[v][s], uq[v]);
   }
-  const float* eg = m.el_geo32;
-  const float xx = eg[G_XX * nes + es], xy = eg[G_XY * nes + es], ex = eg[G_EX * nes + es], ey = eg[G_EY * nes + es];
-  const float j2 = eg[G_2J * nes + es];
   const float hcr = (float)m.hcr;
   // fluxes at the lane's four points, in the metric-transformed form
   float ft[4][4], gt[4][4];  // [var][r]
@@ -665,25 +685,41 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
       acc[v] = mfma4(aDeta[s], gt[v][s], acc[v]);
     }
   }
-  // surface: B operand = signed, 2/J-scaled point flux of (face, point) = 4g + s, point order reversed on the right
-  float sv[4][4];  // [s][var]
+  // surface: B operand = signed, 2/J-scaled point flux of (face, point) = 4g + s, point order reversed on
+  // the right; the four gathers go out together (unused slots read edge 0 and are selected to 0)
+  float4 fl[4];
+  bool use[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int mp = 4 * g + s;
-    float4 fl = make_float4(0.f, 0.f, 0.f, 0.f);
-    float sc = 0.f;
-    if (valid && mp < 3 * NE) {
-      const int i = mp / NE, pt = mp % NE;
-      const int ed = m.el_edge[i * nes + e];
-      const int side = ed & 1;
-      const int src = side ? (NE - 1 - pt) : pt;
-      fl = *reinterpret_cast<const float4*>(G + ((size_t)(ed >> 1) * NE + src) * 4);
-      sc = side ? j2 : -j2;
-    }
-    sv[s][0] = sc * fl.x;
-    sv[s][1] = sc * fl.y;
-    sv[s][2] = sc * fl.z;
-    sv[s][3] = sc * fl.w;
+    use[s] = valid && mp < 3 * NE;
+    const int ed = use[s] ? eid[s] : 0;
+    const int pt = mp % NE;
+    const int src = (ed & 1) ? (NE - 1 - pt) : pt;
+    fl[s] = *reinterpret_cast<const float4*>(G + ((size_t)(ed >> 1) * NE + src) * 4);
+  }
+  // the RK operands of the lane's entries, read after any pending write-back above and before any store
+  float d0[4][4];
+  double un0[4][4];
+  if (OP == OP_S1 || OP == OP_S2) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 4 * g + r;
+        const size_t ix = (size_t)(v * NP + (k < NP ? k : NP - 1)) * nes + es;
+        d0[v][r] = b.d[ix];
+        if (OP == OP_S2) un0[v][r] = b.un[ix];
+      }
+  }
+  float sv[4][4];  // [s][var]
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const float sc = (eid[s] & 1) ? j2 : -j2;
+    sv[s][0] = keep_if(use[s], sc * fl[s].x);
+    sv[s][1] = keep_if(use[s], sc * fl[s].y);
+    sv[s][2] = keep_if(use[s], sc * fl[s].z);
+    sv[s][3] = keep_if(use[s], sc * fl[s].w);
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -701,7 +737,6 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
       }
     return;
   }
-  const double dt = cc.dt;
   const float dtf = (float)dt;
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -717,13 +752,12 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
         b.s[ix] = us[v][r] + d;
       } else if (OP == OP_S1) {
         // u^(2) shadow = u^(1) shadow - d1 + d2 (= s^n + d2 up to float rounding); no s^n re-read
-        const float d0 = b.d[ix];
-        const float d = 0.25f * (d0 + dtf * rr);
+        const float d = 0.25f * (d0[v][r] + dtf * rr);
         b.d[ix] = d;
-        b.s[ix] = (us[v][r] - d0) + d;
+        b.s[ix] = (us[v][r] - d0[v][r]) + d;
       } else {
-        const double d = 2.0 / 3.0 * ((double)b.d[ix] + dt * (double)rr);
-        const double un = b.un[ix] + d;
+        const double d = 2.0 / 3.0 * ((double)d0[v][r] + dt * (double)rr);
+        const double un = un0[v][r] + d;
         b.un[ix] = un;
         b.sn[ix] = (float)un;
       }
