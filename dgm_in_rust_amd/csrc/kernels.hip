@@ -899,15 +899,49 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
   A djump = 0, ejump = 0, surface = 0, minr = A(1.0e10), mine = A(1.0e10);
   const double* phie = lt + LO::PHIE;
   const double* ew = lt + LO::EW;
+  // Connectivity and face geometry of all three faces, then (FP32) the neighbours' density and energy
+  // modes of all three faces, each group issued together rather than face by face. A face without a
+  // neighbour reads the element itself, unused.
+  int nbr[3], eds[3];
 #pragma unroll
   for (int lic = 0; lic < 3; ++lic) {
-    const int nb = m.el_nbr[lic * nes + e];
-    if (nb < 0) continue;
-    const int es = m.el_edge[lic * nes + e];
-    const int edge = es >> 1;
-    const A sgn = (es & 1) ? A(-1) : A(1);
-    const A nx = sgn * (A)m.ed_geo[edge], ny = sgn * (A)m.ed_geo[m.ed_stride + edge];
-    const A len = A(2) * (A)m.ed_geo[2 * m.ed_stride + edge];
+    nbr[lic] = m.el_nbr[lic * nes + e];
+    eds[lic] = m.el_edge[lic * nes + e];
+  }
+  const A h15 = (A)m.el_geo[G_CR15 * nes + e];
+  A fnx[3], fny[3], flen[3];
+#pragma unroll
+  for (int lic = 0; lic < 3; ++lic) {
+    const int edge = eds[lic] >> 1;
+    const A sgn = (eds[lic] & 1) ? A(-1) : A(1);
+    fnx[lic] = sgn * (A)m.ed_geo[edge];
+    fny[lic] = sgn * (A)m.ed_geo[m.ed_stride + edge];
+    flen[lic] = A(2) * (A)m.ed_geo[2 * m.ed_stride + edge];
+  }
+  constexpr bool PRELOAD = sizeof(T) == 4;
+  T nrho[PRELOAD ? 3 : 1][NP], nen[PRELOAD ? 3 : 1][NP];
+  if constexpr (PRELOAD) {
+#pragma unroll
+    for (int lic = 0; lic < 3; ++lic) {
+      const int nb = nbr[lic] >= 0 ? nbr[lic] : e;
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        nrho[lic][k] = U[(size_t)(0 * NP + k) * nes + nb];
+        nen[lic][k] = U[(size_t)(3 * NP + k) * nes + nb];
+      }
+    }
+  }
+  // FP32: every face is evaluated and its terms are selected in (the same operations in the same
+  // order as skipping it), so no branch separates the loads above from their uses. FP64 keeps the
+  // branches, which keep the neighbour loads (and their registers) to the inflow faces.
+#pragma unroll
+  for (int lic = 0; lic < 3; ++lic) {
+    const int nb = nbr[lic];
+    if constexpr (!PRELOAD) {
+      if (nb < 0) continue;
+    }
+    const bool has = nb >= 0;
+    const A nx = fnx[lic], ny = fny[lic], len = flen[lic];
     A tr[NE], te[NE];
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
@@ -920,8 +954,8 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
       }
       tr[q] = a;
       te[q] = b;
-      if (a < minr) minr = a;
-      if (b < mine) mine = b;
+      minr = (has && a < minr) ? a : minr;
+      mine = (has && b < mine) ? b : mine;
     }
     A mu = 0, mv = 0;
 #pragma unroll
@@ -930,7 +964,25 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
       mu += (A)ru[k] * ph;
       mv += (A)rv[k] * ph;
     }
-    if (mu * nx + mv * ny < A(0)) {
+    const bool inflow = has && mu * nx + mv * ny < A(0);
+    if constexpr (PRELOAD) {
+      A rj = 0, ej = 0;
+#pragma unroll
+      for (int q = 0; q < NE; ++q) {
+        A xr = 0, xe = 0;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          const A ph = (A)phie[(lic * NE + (NE - 1 - q)) * NP + k];
+          xr += (A)nrho[lic][k] * ph;
+          xe += (A)nen[lic][k] * ph;
+        }
+        rj += A(0.5) * len * (A)ew[q] * (tr[q] - xr);
+        ej += A(0.5) * len * (A)ew[q] * (te[q] - xe);
+      }
+      djump = inflow ? djump + rj : djump;
+      ejump = inflow ? ejump + ej : ejump;
+      surface = inflow ? surface + len : surface;
+    } else if (inflow) {
       A rj = 0, ej = 0;
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
@@ -949,7 +1001,6 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
       surface += len;
     }
   }
-  const A h15 = (A)m.el_geo[G_CR15 * nes + e];
   const A id = fabs(djump) / (h15 * surface * minr);
   const A ie = fabs(ejump) / (h15 * surface * mine);
   return (id >= A(1)) || (ie >= A(1));
@@ -985,12 +1036,13 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
   bool t = false;
   double dt = 1.0e10;
   if (valid) {
+    const double hmin = m.el_geo[G_HMIN * m.nes + e], cfl = cc->cfl;
     double s0, s1, s2, s3;
     t = kxrcf<T, P, NQ, NE>(m, U, lt, e, s0, s1, s2, s3);
     if (lb.flags) lb.flags[e] = t ? 1 : 0;
     if (!t) {
       const double d = element_dt<1>([&](int v, int) { return v == 0 ? s0 : (v == 1 ? s1 : (v == 2 ? s2 : s3)); },
-                                     m.hcr, cc->cfl, m.el_geo[G_HMIN * m.nes + e]);
+                                     m.hcr, cfl, hmin);
       if (d < dt) dt = d;
     }
   }
