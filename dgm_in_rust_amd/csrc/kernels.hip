@@ -499,27 +499,57 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
     return;
   }
   const double dt = cc.dt;
+  // The RK operands of one variable are read together before its stores (loads after a store to
+  // another array would otherwise each wait for the one before).
   if constexpr (sizeof(T) == 8) {
     // src/temporal_disc/rk3.rs:10,16,22
+    // batches of half a variable, kept below the flux gather (mixed into it they raise the register
+    // count above what keeps two waves per SIMD)
+    constexpr int HB = (NP + 1) / 2;
+    __builtin_amdgcn_sched_barrier(0);
+    if constexpr (OP == OP_S0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
+      for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int k = 0; k < NP; ++k) {
-        const size_t ix = (size_t)(v * NP + k) * nes + base;
+        for (int k = 0; k < NP; ++k) b.us[(size_t)(v * NP + k) * nes + base] = (double)u[v][k] + dt * (double)r[v][k];
+    } else
+#pragma unroll
+    for (int c0 = 0; c0 < 4 * NP; c0 += HB) {
+      double un0[HB];
+      if (OP != OP_S0) {
+#pragma unroll
+        for (int c = 0; c < HB; ++c)
+          if (c0 + c < 4 * NP) un0[c] = b.un[(size_t)(c0 + c) * nes + base];
+      }
+#pragma unroll
+      for (int c = 0; c < HB; ++c) {
+        if (c0 + c >= 4 * NP) break;
+        const int v = (c0 + c) / NP, k = (c0 + c) % NP;
+        const size_t ix = (size_t)(c0 + c) * nes + base;
         const double rr = (double)r[v][k], uu = (double)u[v][k];
         if (OP == OP_S0)
           b.us[ix] = uu + dt * rr;
         else if (OP == OP_S1)
-          b.us[ix] = 0.75 * b.un[ix] + 0.25 * (uu + dt * rr);
+          b.us[ix] = 0.75 * un0[c] + 0.25 * (uu + dt * rr);
         else
-          b.un[ix] = 1.0 / 3.0 * b.un[ix] + 2.0 / 3.0 * (uu + dt * rr);
+          b.un[ix] = 1.0 / 3.0 * un0[c] + 2.0 / 3.0 * (uu + dt * rr);
       }
+    }
   } else {
     // FP32-bulk in increment form: u^(1) = u^n + d1, d1 = dt R(u^n); u^(2) = u^n + d2,
     // d2 = (d1 + dt R(u^(1))) / 4; u^(n+1) = u^n + 2 (d2 + dt R(u^(2))) / 3 accumulated in FP64.
     const float dtf = (float)dt;
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
+    for (int v = 0; v < 4; ++v) {
+      float d0[NP];
+      double un0[NP];
+      if (OP != OP_S0) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          d0[k] = b.d[(size_t)(v * NP + k) * nes + base];
+          if (OP == OP_S2) un0[k] = b.un[(size_t)(v * NP + k) * nes + base];
+        }
+      }
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
         const size_t ix = (size_t)(v * NP + k) * nes + base;
@@ -529,17 +559,17 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
           b.d[ix] = d;
           b.s[ix] = (float)u[v][k] + d;
         } else if (OP == OP_S1) {
-          const float d0 = b.d[ix];
-          const float d = 0.25f * (d0 + dtf * rr);
+          const float d = 0.25f * (d0[k] + dtf * rr);
           b.d[ix] = d;
-          b.s[ix] = ((float)u[v][k] - d0) + d;
+          b.s[ix] = ((float)u[v][k] - d0[k]) + d;
         } else {
-          const double d = 2.0 / 3.0 * ((double)b.d[ix] + dt * (double)rr);
-          const double un = b.un[ix] + d;
+          const double d = 2.0 / 3.0 * ((double)d0[k] + dt * (double)rr);
+          const double un = un0[k] + d;
           b.un[ix] = un;
           b.sn[ix] = (float)un;
         }
       }
+    }
   }
 }
 
@@ -1206,7 +1236,7 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
     double w = 0.0;
 #pragma unroll
     for (int k = 0; k < NP; ++k) Pk[k] = 0.0;
-    if (sh.nb[i] >= 0) {
+    if (nbr[i] >= 0) {
       const double nx = fnx, ny = fny;
       const double mean[4] = {sh.src[0][0][0], sh.src[0][1][0], sh.src[0][2][0], sh.src[0][3][0]};
       double Lm[16], Rm[16];
@@ -1228,7 +1258,7 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
 #pragma unroll
         for (int c = 0; c < 4; ++c) sh.Rrow[i][v][c] = rr[c];
       }
-      if (j == 0 || sh.nb[j - 1] >= 0) {
+      if (j == 0 || nbr[j - 1] >= 0) {
 #pragma unroll
         for (int k = 0; k < NP; ++k)
           Pk[k] = lr[0] * sh.src[j][0][k] + lr[1] * sh.src[j][1][k] + lr[2] * sh.src[j][2][k] + lr[3] * sh.src[j][3][k];
@@ -1251,27 +1281,24 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
     sh.nlw[i][j][v] = w;
   }
   __syncthreads();
-  if (lane < 12) {
-    const int i = lane >> 2, v = lane & 3;
-    if (sh.nb[i] >= 0) {
-      const double den = ((sh.nlw[i][0][v] + sh.nlw[i][1][v]) + sh.nlw[i][2][v]) + sh.nlw[i][3][v];
-      double wj[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) wj[j] = sh.nlw[i][j][v] / den;
-#pragma unroll
-      for (int k = 0; k < NP; ++k)
-        sh.PN[i][v][k] = wj[0] * sh.Pj[i][0][v][k] + wj[1] * sh.Pj[i][1][v][k] + wj[2] * sh.Pj[i][2][v][k] +
-                         wj[3] * sh.Pj[i][3][v][k];
+  // the two combinations below run over (face, variable, mode) items, one LDS round trip per item
+  for (int it = lane; it < 12 * NP; it += 64) {
+    const int i = it / (4 * NP), v = (it / NP) % 4, k = it % NP;
+    if (nbr[i] >= 0) {
+      const double n0 = sh.nlw[i][0][v], n1 = sh.nlw[i][1][v], n2 = sh.nlw[i][2][v], n3 = sh.nlw[i][3][v];
+      const double p0 = sh.Pj[i][0][v][k], p1 = sh.Pj[i][1][v][k], p2 = sh.Pj[i][2][v][k], p3 = sh.Pj[i][3][v][k];
+      const double den = ((n0 + n1) + n2) + n3;
+      const double w0 = n0 / den, w1 = n1 / den, w2 = n2 / den, w3 = n3 / den;
+      sh.PN[i][v][k] = w0 * p0 + w1 * p1 + w2 * p2 + w3 * p3;
     }
   }
   __syncthreads();
-  if (lane < 12) {
-    const int i = lane >> 2, v = lane & 3;
-    if (sh.nb[i] >= 0) {
-#pragma unroll
-      for (int k = 0; k < NP; ++k)
-        sh.RES[i][v][k] = sh.Rrow[i][v][0] * sh.PN[i][0][k] + sh.Rrow[i][v][1] * sh.PN[i][1][k] +
-                          sh.Rrow[i][v][2] * sh.PN[i][2][k] + sh.Rrow[i][v][3] * sh.PN[i][3][k];
+  for (int it = lane; it < 12 * NP; it += 64) {
+    const int i = it / (4 * NP), v = (it / NP) % 4, k = it % NP;
+    if (nbr[i] >= 0) {
+      const double r0 = sh.Rrow[i][v][0], r1 = sh.Rrow[i][v][1], r2 = sh.Rrow[i][v][2], r3 = sh.Rrow[i][v][3];
+      const double q0 = sh.PN[i][0][k], q1 = sh.PN[i][1][k], q2 = sh.PN[i][2][k], q3 = sh.PN[i][3][k];
+      sh.RES[i][v][k] = r0 * q0 + r1 * q1 + r2 * q2 + r3 * q3;
     }
   }
   __syncthreads();
@@ -1280,7 +1307,7 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
     const int v = idx / NP, k = idx % NP;
     double rec = 0.0, area = 0.0;
     for (int i = 0; i < 3; ++i)
-      if (sh.nb[i] >= 0) {
+      if (nbr[i] >= 0) {
         rec += sh.RES[i][v][k] * sh.area[i];
         area += sh.area[i];
       }
@@ -1303,15 +1330,21 @@ __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U,
     const int cs = case_of(m, m.el_case64, e);
     CaseCtl* cc = ctl.cases + cs;
     double* out = lb.w + (size_t)s * 4 * NP;
+    // what the fused time step needs is read before the rebuild, whose barriers keep it there
+    double hmin = 0.0, cfl = 0.0;
+    int step = 0;
+    if (fuse_dt) {
+      hmin = m.el_geo[G_HMIN * m.nes + e];
+      cfl = cc->cfl;
+      step = cc->step;
+    }
     const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, e, sh, out);
     if (st && threadIdx.x == 0) atomicMin(&cc->error, (0xFFFFFFull << 8) | (unsigned long long)st);
     __syncthreads();
     if (fuse_dt && !st && threadIdx.x == 0) {
-      const double d = element_dt<NP>([&](int v, int k) { return sh.fin[v * NP + k]; }, m.hcr, cc->cfl,
-                                      m.el_geo[G_HMIN * m.nes + e]);
+      const double d = element_dt<NP>([&](int v, int k) { return sh.fin[v * NP + k]; }, m.hcr, cfl, hmin);
       if (d < 1.0e10)
-        atomicMin(dt_slot(ctl, cs, cc->step + 1) + (e % DT_LANES) * DT_STRIDE,
-                  (unsigned long long)__double_as_longlong(d));
+        atomicMin(dt_slot(ctl, cs, step + 1) + (e % DT_LANES) * DT_STRIDE, (unsigned long long)__double_as_longlong(d));
     }
     __syncthreads();
   }
