@@ -228,6 +228,16 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
     err = "arena capacity beyond the 32-bit index range";
     return DGM_BAD_CONFIG;
   }
+  // The kernels address one variable's block of modes ([Np][cap_e], up to 8 bytes each) and the point
+  // fluxes ([cap_ed][Ne][4]) with 32-bit byte offsets from a base pointer.
+  {
+    const uint64_t np = (uint64_t)(order + 1) * (order + 2) / 2, lim = 1ull << 32;
+    const uint64_t flux_bytes = (uint64_t)cap_ed * ne_pts * 4 * (precision == DGM_FP32_BULK ? 4 : 8);
+    if (np * (uint64_t)cap_e * 8 >= lim || flux_bytes >= lim) {
+      err = "arena capacity beyond the 32-bit offset range of one variable block or of the flux array";
+      return DGM_BAD_CONFIG;
+    }
+  }
   r_e.reset(cap_e / 64);
   r_ed.reset(cap_ed / 64);
   r_patch.reset(cap_patch);

@@ -237,6 +237,18 @@ __device__ __forceinline__ void farfield_flux(const T lv[4], const double* qinf,
   out[3] = (q[3] + pn) * vnn;
 }
 
+// Element-array access through a 64-bit base in scalar registers plus a 32-bit unsigned byte offset per
+// lane, which the memory instruction adds itself: no 64-bit address arithmetic per lane. The byte offset
+// must stay below 4 GiB; Arena::init rejects capacities whose variable blocks or flux array would not.
+template <class T>
+__device__ __forceinline__ T ld32(const T* base, unsigned idx) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + idx * (unsigned)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ void st32(T* base, unsigned idx, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (unsigned)sizeof(T)) = v;
+}
+
 __device__ __forceinline__ void record_error(CaseCtl* cc, int edge, int code) {
   atomicMin(&cc->error, ((unsigned long long)(unsigned)edge << 8) | (unsigned long long)code);
 }
@@ -425,7 +437,7 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
 #pragma unroll
   for (int v = 0; v < 4; ++v)
 #pragma unroll
-    for (int k = 0; k < NP; ++k) u[v][k] = b.in[(size_t)(v * NP + k) * nes + e];
+    for (int k = 0; k < NP; ++k) u[v][k] = ld32(b.in + (size_t)v * NP * nes, (unsigned)(k * nes + e));
   // a rebuild still pending from the limiter pass before: use it, and write it back (rare)
   if (OP != OP_RESID && b.pend) {
     const int sl = b.pend[e];
@@ -442,8 +454,10 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
     }
   }
   const T* eg = geo_el<T>(m);
-  const T xx = eg[G_XX * nes + e], xy = eg[G_XY * nes + e], ex = eg[G_EX * nes + e], ey = eg[G_EY * nes + e];
-  const T j2 = eg[G_2J * nes + e];
+  const unsigned eu = (unsigned)e;
+  const T xx = ld32(eg + (size_t)G_XX * nes, eu), xy = ld32(eg + (size_t)G_XY * nes, eu);
+  const T ex = ld32(eg + (size_t)G_EX * nes, eu), ey = ld32(eg + (size_t)G_EY * nes, eu);
+  const T j2 = ld32(eg + (size_t)G_2J * nes, eu);
   const T hcr = (T)m.hcr;
   T r[4][NP];
 #pragma unroll
@@ -474,28 +488,27 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   // surface: gather the three edges' point fluxes (reversed point order on the right side)
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    const int es = m.el_edge[i * nes + e];
+    const int es = ld32(m.el_edge, (unsigned)(i * nes + e));
     const int edge = es >> 1, side = es & 1;
-    const T* g = G + (size_t)edge * NE * 4;
+    const unsigned gb = (unsigned)edge * (unsigned)(NE * 4);
     const T sg = side ? j2 : -j2;
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
       const int src = side ? (NE - 1 - q) : q;
       T gv[4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) gv[v] = sg * g[src * 4 + v];
+      for (int v = 0; v < 4; ++v) gv[v] = sg * ld32(G, gb + src * 4 + v);
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
         for (int k = 0; k < NP; ++k) r[v][k] += gv[v] * tab[O::LIFT + (i * NE + q) * NP + k];
     }
   }
-  const size_t base = e;
   if (OP == OP_RESID) {
 #pragma unroll
     for (int v = 0; v < 4; ++v)
 #pragma unroll
-      for (int k = 0; k < NP; ++k) b.rout[(size_t)(v * NP + k) * nes + base] = (double)r[v][k];
+      for (int k = 0; k < NP; ++k) st32(b.rout + (size_t)v * NP * nes, (unsigned)(k * nes + e), (double)r[v][k]);
     return;
   }
   const double dt = cc.dt;
@@ -511,7 +524,8 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
-        for (int k = 0; k < NP; ++k) b.us[(size_t)(v * NP + k) * nes + base] = (double)u[v][k] + dt * (double)r[v][k];
+        for (int k = 0; k < NP; ++k)
+          st32(b.us + (size_t)v * NP * nes, (unsigned)(k * nes + e), (double)u[v][k] + dt * (double)r[v][k]);
     } else
 #pragma unroll
     for (int c0 = 0; c0 < 4 * NP; c0 += HB) {
@@ -519,20 +533,21 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
       if (OP != OP_S0) {
 #pragma unroll
         for (int c = 0; c < HB; ++c)
-          if (c0 + c < 4 * NP) un0[c] = b.un[(size_t)(c0 + c) * nes + base];
+          if (c0 + c < 4 * NP) un0[c] = ld32(b.un + (size_t)((c0 + c) / NP) * NP * nes, (unsigned)(((c0 + c) % NP) * nes + e));
       }
 #pragma unroll
       for (int c = 0; c < HB; ++c) {
         if (c0 + c >= 4 * NP) break;
         const int v = (c0 + c) / NP, k = (c0 + c) % NP;
-        const size_t ix = (size_t)(c0 + c) * nes + base;
+        const size_t vb = (size_t)v * NP * nes;
+        const unsigned ix = (unsigned)(k * nes + e);
         const double rr = (double)r[v][k], uu = (double)u[v][k];
         if (OP == OP_S0)
-          b.us[ix] = uu + dt * rr;
+          st32(b.us + vb, ix, uu + dt * rr);
         else if (OP == OP_S1)
-          b.us[ix] = 0.75 * un0[c] + 0.25 * (uu + dt * rr);
+          st32(b.us + vb, ix, 0.75 * un0[c] + 0.25 * (uu + dt * rr));
         else
-          b.un[ix] = 1.0 / 3.0 * un0[c] + 2.0 / 3.0 * (uu + dt * rr);
+          st32(b.un + vb, ix, 1.0 / 3.0 * un0[c] + 2.0 / 3.0 * (uu + dt * rr));
       }
     }
   } else {
@@ -546,27 +561,28 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
       if (OP != OP_S0) {
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
-          d0[k] = b.d[(size_t)(v * NP + k) * nes + base];
-          if (OP == OP_S2) un0[k] = b.un[(size_t)(v * NP + k) * nes + base];
+          d0[k] = ld32(b.d + (size_t)v * NP * nes, (unsigned)(k * nes + e));
+          if (OP == OP_S2) un0[k] = ld32(b.un + (size_t)v * NP * nes, (unsigned)(k * nes + e));
         }
       }
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        const size_t ix = (size_t)(v * NP + k) * nes + base;
+        const size_t vb = (size_t)v * NP * nes;
+        const unsigned ix = (unsigned)(k * nes + e);
         const float rr = (float)r[v][k];
         if (OP == OP_S0) {
           const float d = dtf * rr;
-          b.d[ix] = d;
-          b.s[ix] = (float)u[v][k] + d;
+          st32(b.d + vb, ix, d);
+          st32(b.s + vb, ix, (float)u[v][k] + d);
         } else if (OP == OP_S1) {
           const float d = 0.25f * (d0[k] + dtf * rr);
-          b.d[ix] = d;
-          b.s[ix] = ((float)u[v][k] - d0[k]) + d;
+          st32(b.d + vb, ix, d);
+          st32(b.s + vb, ix, ((float)u[v][k] - d0[k]) + d);
         } else {
           const double d = 2.0 / 3.0 * ((double)d0[k] + dt * (double)rr);
           const double un = un0[k] + d;
-          b.un[ix] = un;
-          b.sn[ix] = (float)un;
+          st32(b.un + vb, ix, un);
+          st32(b.sn + vb, ix, (float)un);
         }
       }
     }
@@ -599,7 +615,7 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
 }
 
 template <int NP, int NQ, int NE, int OP>
-__global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float> b, const float* __restrict__ G,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_element_mfma(DevMesh m, StageBufs<float> b, const float* __restrict__ G,
                                                       const float* __restrict__ tab, Ctl ctl, int force, int remap) {
   static_assert(NP <= 16 && NQ <= 16 && 3 * NE <= 16, "matrix-core element kernel needs p <= 3");
   using O = TabOff<NP, NQ, NE>;
@@ -612,6 +628,11 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
   const int cs = case_of(m, m.el_case64, wave * 16);
   if (cs < 0) return;
   const CaseCtl& cc = ctl.cases[cs];
+  // the fields of the case record this kernel uses, read first: the exit test below then waits on
+  // them alone, not on the loads issued after them
+  const int c_active = cc.active, c_live = cc.live, e_end = cc.e_end;
+  const unsigned long long c_error = cc.error;
+  const double dt = cc.dt;
   const int c = lane & 15, g = lane >> 4;
   const int e = wave * 16 + c;
   const int nes = m.nes;
@@ -619,6 +640,14 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
   // afterwards, so the loads carry no branches and are issued together: the state, the table
   // operands, the edge ids of the lift gather and the metric go out before the case record is back.
   const int es = e < m.ne ? e : 0;
+  // edge ids of the lane's (face, point) slots 4g + s of the lift product, first: the flux gather waits
+  // on them alone
+  int eid[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int mp = 4 * g + s;
+    eid[s] = ld32(m.el_edge, (unsigned)((mp < 3 * NE ? mp / NE : 0) * nes + es));
+  }
   // state entries held by this lane: U_v[e][k = 4g + s]
   float us[4][4];
 #pragma unroll
@@ -626,43 +655,51 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int k = 4 * g + s;
-      const float x = b.in[(size_t)(v * NP + (k < NP ? k : NP - 1)) * nes + es];
+      const float x = ld32(b.in + (size_t)v * NP * nes, (unsigned)((k < NP ? k : NP - 1) * nes + es));
       us[v][s] = keep_if(k < NP, x);
     }
-  // per-lane table operands (A of each product)
-  float aPhi[4], aDxi[4], aDeta[4], aLift[4];
+  // per-lane table operands (A of the value and volume products; the lift operands are read later)
+  float aPhi[4], aDxi[4], aDeta[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    const int k = 4 * g + s;  // value product: K index = mode; volume: K index = point; lift: face * NE + point
-    const int kp = k < NP ? k : NP - 1, kq = k < NQ ? k : NQ - 1, km = k < 3 * NE ? k : 3 * NE - 1;
+    const int k = 4 * g + s;  // value product: K index = mode; volume: K index = point
+    const int kp = k < NP ? k : NP - 1, kq = k < NQ ? k : NQ - 1;
     const int cq = c < NQ ? c : NQ - 1, cp = c < NP ? c : NP - 1;
-    const float ph = tab[O::PHIV + cq * NP + kp];
-    const float dx = tab[O::DXI + kq * NP + cp], de = tab[O::DETA + kq * NP + cp];
-    const float li = tab[O::LIFT + km * NP + cp];
+    const float ph = ld32(tab, (unsigned)(O::PHIV + cq * NP + kp));
+    const float dx = ld32(tab, (unsigned)(O::DXI + kq * NP + cp)), de = ld32(tab, (unsigned)(O::DETA + kq * NP + cp));
     aPhi[s] = keep_if(k < NP && c < NQ, ph);
     aDxi[s] = keep_if(k < NQ && c < NP, dx);
     aDeta[s] = keep_if(k < NQ && c < NP, de);
-    aLift[s] = keep_if(k < 3 * NE && c < NP, li);
-  }
-  // edge ids of the lane's (face, point) slots 4g + s of the lift product
-  int eid[4];
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int mp = 4 * g + s;
-    eid[s] = m.el_edge[(mp < 3 * NE ? mp / NE : 0) * nes + es];
   }
   const float* eg = m.el_geo32;
-  const float xx = eg[G_XX * nes + es], xy = eg[G_XY * nes + es], ex = eg[G_EX * nes + es], ey = eg[G_EY * nes + es];
-  const float j2 = eg[G_2J * nes + es];
-  const int pend_slot = (OP != OP_RESID && b.pend) ? b.pend[es] : -1;
-  if (!force && !case_runs(cc)) {
+  const unsigned eu = (unsigned)es;
+  const float xx = ld32(eg + (size_t)G_XX * nes, eu), xy = ld32(eg + (size_t)G_XY * nes, eu);
+  const float ex = ld32(eg + (size_t)G_EX * nes, eu), ey = ld32(eg + (size_t)G_EY * nes, eu);
+  const float j2 = ld32(eg + (size_t)G_2J * nes, eu);
+  const int pend_slot = (OP != OP_RESID && b.pend) ? ld32(b.pend, eu) : -1;
+  const bool runs = (force != 0) | ((c_active != 0) & (c_error == ~0ull));  // case_runs()
+  if (!runs) {
     const int ew = wave * 16 + (lane & 15);
-    if (OP == OP_S0 && b.pend && cc.live && lane < 16 && ew < m.ne && ew < cc.e_end)
+    if (OP == OP_S0 && b.pend && c_live && lane < 16 && ew < m.ne && ew < e_end)
       write_back_pending<float, NP>(b, ew, m.nes);
     return;
   }
-  const bool valid = e < m.ne && e < cc.e_end;
-  const double dt = cc.dt;
+  const bool valid = (e < m.ne) & (e < e_end);
+  // surface: B operand = signed, 2/J-scaled point flux of (face, point) = 4g + s, point order reversed on
+  // the right. The four gathers go out together as soon as the edge ids are in, ahead of the volume
+  // product (unused slots read edge 0 and are masked to 0 when used).
+  float4 fl[4];
+  bool use[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int mp = 4 * g + s;
+    use[s] = valid && mp < 3 * NE;
+    // the address depends on the edge ids only (not on the case record): any in-range element's ids
+    const int ed = (e < m.ne && mp < 3 * NE) ? eid[s] : 0;
+    const int pt = mp % NE;
+    const int src = (ed & 1) ? (NE - 1 - pt) : pt;
+    fl[s] = ld32(reinterpret_cast<const float4*>(G), (unsigned)((ed >> 1) * NE + src));
+  }
   // a rebuild still pending from the limiter pass before: use it and write it back, each lane the
   // modes it holds (rare)
   if (OP != OP_RESID && b.pend && valid) {
@@ -715,19 +752,6 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
       acc[v] = mfma4(aDeta[s], gt[v][s], acc[v]);
     }
   }
-  // surface: B operand = signed, 2/J-scaled point flux of (face, point) = 4g + s, point order reversed on
-  // the right; the four gathers go out together (unused slots read edge 0 and are selected to 0)
-  float4 fl[4];
-  bool use[4];
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int mp = 4 * g + s;
-    use[s] = valid && mp < 3 * NE;
-    const int ed = use[s] ? eid[s] : 0;
-    const int pt = mp % NE;
-    const int src = (ed & 1) ? (NE - 1 - pt) : pt;
-    fl[s] = *reinterpret_cast<const float4*>(G + ((size_t)(ed >> 1) * NE + src) * 4);
-  }
   // the RK operands of the lane's entries, read after any pending write-back above and before any store
   float d0[4][4];
   double un0[4][4];
@@ -737,10 +761,18 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int k = 4 * g + r;
-        const size_t ix = (size_t)(v * NP + (k < NP ? k : NP - 1)) * nes + es;
-        d0[v][r] = b.d[ix];
-        if (OP == OP_S2) un0[v][r] = b.un[ix];
+        const unsigned ix = (unsigned)((k < NP ? k : NP - 1) * nes + es);
+        d0[v][r] = ld32(b.d + (size_t)v * NP * nes, ix);
+        if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
       }
+  }
+  // lift operands: K index = face * NE + point
+  float aLift[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = 4 * g + s;
+    const int km = k < 3 * NE ? k : 3 * NE - 1, cp = c < NP ? c : NP - 1;
+    aLift[s] = keep_if(k < 3 * NE && c < NP, ld32(tab, (unsigned)(O::LIFT + km * NP + cp)));
   }
   float sv[4][4];  // [s][var]
 #pragma unroll
@@ -763,7 +795,7 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int k = 4 * g + r;
-        if (k < NP) b.rout[(size_t)(v * NP + k) * nes + e] = (double)acc[v][r];
+        if (k < NP) st32(b.rout + (size_t)v * NP * nes, (unsigned)(k * nes + e), (double)acc[v][r]);
       }
     return;
   }
@@ -774,22 +806,23 @@ __global__ __launch_bounds__(256) void k_element_mfma(DevMesh m, StageBufs<float
     for (int r = 0; r < 4; ++r) {
       const int k = 4 * g + r;
       if (k >= NP) continue;
-      const size_t ix = (size_t)(v * NP + k) * nes + e;
+      const unsigned ix = (unsigned)(k * nes + e);
+      const size_t vb = (size_t)v * NP * nes;
       const float rr = acc[v][r];
       if (OP == OP_S0) {
         const float d = dtf * rr;
-        b.d[ix] = d;
-        b.s[ix] = us[v][r] + d;
+        st32(b.d + vb, ix, d);
+        st32(b.s + vb, ix, us[v][r] + d);
       } else if (OP == OP_S1) {
         // u^(2) shadow = u^(1) shadow - d1 + d2 (= s^n + d2 up to float rounding); no s^n re-read
         const float d = 0.25f * (d0[v][r] + dtf * rr);
-        b.d[ix] = d;
-        b.s[ix] = (us[v][r] - d0[v][r]) + d;
+        st32(b.d + vb, ix, d);
+        st32(b.s + vb, ix, (us[v][r] - d0[v][r]) + d);
       } else {
         const double d = 2.0 / 3.0 * ((double)d0[v][r] + dt * (double)rr);
         const double un = un0[v][r] + d;
-        b.un[ix] = un;
-        b.sn[ix] = (float)un;
+        st32(b.un + vb, ix, un);
+        st32(b.sn + vb, ix, (float)un);
       }
     }
 }
@@ -906,10 +939,11 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
   T rho[NP], ru[NP], rv[NP], en[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
-    rho[k] = U[(size_t)(0 * NP + k) * nes + e];
-    ru[k] = U[(size_t)(1 * NP + k) * nes + e];
-    rv[k] = U[(size_t)(2 * NP + k) * nes + e];
-    en[k] = U[(size_t)(3 * NP + k) * nes + e];
+    const unsigned o = (unsigned)(k * nes + e);
+    rho[k] = ld32(U, o);
+    ru[k] = ld32(U + (size_t)NP * nes, o);
+    rv[k] = ld32(U + (size_t)2 * NP * nes, o);
+    en[k] = ld32(U + (size_t)3 * NP * nes, o);
   }
   {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -935,18 +969,18 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
   int nbr[3], eds[3];
 #pragma unroll
   for (int lic = 0; lic < 3; ++lic) {
-    nbr[lic] = m.el_nbr[lic * nes + e];
-    eds[lic] = m.el_edge[lic * nes + e];
+    nbr[lic] = ld32(m.el_nbr, (unsigned)(lic * nes + e));
+    eds[lic] = ld32(m.el_edge, (unsigned)(lic * nes + e));
   }
-  const A h15 = (A)m.el_geo[G_CR15 * nes + e];
+  const A h15 = (A)ld32(m.el_geo + (size_t)G_CR15 * nes, (unsigned)e);
   A fnx[3], fny[3], flen[3];
 #pragma unroll
   for (int lic = 0; lic < 3; ++lic) {
     const int edge = eds[lic] >> 1;
     const A sgn = (eds[lic] & 1) ? A(-1) : A(1);
-    fnx[lic] = sgn * (A)m.ed_geo[edge];
-    fny[lic] = sgn * (A)m.ed_geo[m.ed_stride + edge];
-    flen[lic] = A(2) * (A)m.ed_geo[2 * m.ed_stride + edge];
+    fnx[lic] = sgn * (A)ld32(m.ed_geo, (unsigned)edge);
+    fny[lic] = sgn * (A)ld32(m.ed_geo + m.ed_stride, (unsigned)edge);
+    flen[lic] = A(2) * (A)ld32(m.ed_geo + 2 * (size_t)m.ed_stride, (unsigned)edge);
   }
   constexpr bool PRELOAD = sizeof(T) == 4;
   T nrho[PRELOAD ? 3 : 1][NP], nen[PRELOAD ? 3 : 1][NP];
@@ -956,8 +990,8 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
       const int nb = nbr[lic] >= 0 ? nbr[lic] : e;
 #pragma unroll
       for (int k = 0; k < NP; ++k) {
-        nrho[lic][k] = U[(size_t)(0 * NP + k) * nes + nb];
-        nen[lic][k] = U[(size_t)(3 * NP + k) * nes + nb];
+        nrho[lic][k] = ld32(U, (unsigned)(k * nes + nb));
+        nen[lic][k] = ld32(U + (size_t)3 * NP * nes, (unsigned)(k * nes + nb));
       }
     }
   }
@@ -1020,8 +1054,8 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
 #pragma unroll
         for (int k = 0; k < NP; ++k) {
           const A ph = (A)phie[(lic * NE + (NE - 1 - q)) * NP + k];
-          xr += (A)U[(size_t)(0 * NP + k) * nes + nb] * ph;
-          xe += (A)U[(size_t)(3 * NP + k) * nes + nb] * ph;
+          xr += (A)ld32(U, (unsigned)(k * nes + nb)) * ph;
+          xe += (A)ld32(U + (size_t)3 * NP * nes, (unsigned)(k * nes + nb)) * ph;
         }
         rj += A(0.5) * len * (A)ew[q] * (tr[q] - xr);
         ej += A(0.5) * len * (A)ew[q] * (te[q] - xe);
@@ -1066,7 +1100,7 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
   bool t = false;
   double dt = 1.0e10;
   if (valid) {
-    const double hmin = m.el_geo[G_HMIN * m.nes + e], cfl = cc->cfl;
+    const double hmin = ld32(m.el_geo + (size_t)G_HMIN * m.nes, (unsigned)e), cfl = cc->cfl;
     double s0, s1, s2, s3;
     t = kxrcf<T, P, NQ, NE>(m, U, lt, e, s0, s1, s2, s3);
     if (lb.flags) lb.flags[e] = t ? 1 : 0;

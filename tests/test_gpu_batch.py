@@ -96,6 +96,18 @@ def test_admission_by_capacity():
     b.close()
 
 
+def test_capacity_beyond_offset_range_is_rejected():
+    """The kernels address a variable block of modes and the point-flux array with 32-bit byte offsets:
+    an arena whose flux array would reach 4 GiB (p = 3, FP32: 40 M elements -> 64 M edges x 80 bytes)
+    is refused with BAD_CONFIG before anything is allocated; a large arena within range is accepted."""
+    from dgm_in_rust_amd._lib import DgmError
+    with pytest.raises(DgmError) as ex:
+        Batch({3: 40_000_000}, precision="fp32", limiter="jacobi", max_cases=4)
+    assert ex.value.status == 4
+    b = Batch({3: 1_000_000}, precision="fp32", limiter="jacobi", max_cases=4)
+    b.close()
+
+
 def test_per_case_error_is_isolated():
     """A case whose state fails the positivity check stops with its own status; the others go on
     and match their stand-alone runs."""
