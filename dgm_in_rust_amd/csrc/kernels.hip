@@ -348,22 +348,26 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
                                                  const int* __restrict__ pend, const double* __restrict__ pw) {
   using O = TabOff<NP, NQ, NE>;
   __shared__ T sphi[3 * NE * NP];
-  for (int i = threadIdx.x; i < 3 * NE * NP; i += blockDim.x) sphi[i] = tab[O::PHIE + i];
-  __syncthreads();
   const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int ed = blk * blockDim.x + threadIdx.x;
-  if (ed >= m.nedge) return;
-  const int cs = case_of(m, m.ed_case64, ed);
-  if (cs < 0) return;
-  CaseCtl* cc = ctl.cases + cs;
-  if (!force && !case_runs(*cc)) return;
-  if (ed >= cc->ed_end) return;
   const int nes = m.nes, est = m.ed_stride;
-  const int info = m.ed_info[ed];
-  const int licL = info & 3, licR = (info >> 2) & 3, bc = (info >> 4) & 3, patch = info >> 8;
-  const int L = m.ed_elem[ed];
+  // the edge record (read at a clamped index) goes out before the table staging and its barrier
+  const int ee = ed < m.nedge ? ed : 0;
+  const int info = m.ed_info[ee];
+  const int L = m.ed_elem[ee];
   const T* eg = geo_ed<T>(m);
-  const T nx = eg[ed], ny = eg[est + ed], hl = eg[2 * est + ed];
+  const T nx = eg[ee], ny = eg[est + ee], hl = eg[2 * est + ee];
+  // and so does the case record (of case 0 for edges outside any case, which exit below)
+  const int cs = case_of(m, m.ed_case64, ee);
+  CaseCtl* cc = ctl.cases + (cs < 0 ? 0 : cs);
+  const int c_active = cc->active, c_ed_end = cc->ed_end;
+  const unsigned long long c_error = cc->error;
+  for (int i = threadIdx.x; i < 3 * NE * NP; i += blockDim.x) sphi[i] = tab[O::PHIE + i];
+  __syncthreads();
+  if (ed >= m.nedge || cs < 0) return;
+  if (!force && !(c_active && c_error == ~0ull)) return;  // case_runs()
+  if (ed >= c_ed_end) return;
+  const int licL = info & 3, licR = (info >> 2) & 3, bc = (info >> 4) & 3, patch = info >> 8;
   const T hcr = (T)m.hcr;
   // an element rebuilt by the last limiter pass and not yet written back is read from the rebuild
   // (rare: the stored modes are loaded regardless, so the common path waits on no extra load)
