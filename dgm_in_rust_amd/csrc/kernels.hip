@@ -934,8 +934,9 @@ __global__ void k_reset_count(Ctl ctl, int slot) { ctl.count[slot] = 0; }
 // (FP64 in FP64 mode; FP32 in FP32-bulk mode, where only the WENO indicators and weights and the
 // positivity tests stay in FP64); the mode sums for the next time step are FP64 in both.
 template <class T, int P, int NQ, int NE>
-__device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt, int e,
-                                      double& sum0, double& sum1, double& sum2, double& sum3) {
+__device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt,
+                                      const T* __restrict__ tphie, int e, double& sum0, double& sum1, double& sum2,
+                                      double& sum3) {
   using LO = LimOff<P, NQ, NE>;
   constexpr int NP = LO::NP;
   const int nes = m.nes;
@@ -965,7 +966,9 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
   }
   using A = T;  // detection arithmetic in the bulk type
   A djump = 0, ejump = 0, surface = 0, minr = A(1.0e10), mine = A(1.0e10);
-  const double* phie = lt + LO::PHIE;
+  // edge basis values [3][NE][NP] in the bulk type (the same values as the FP64 table, rounded once on the
+  // host in FP32-bulk mode, so the detector converts nothing)
+  const T* phie = tphie;
   const double* ew = lt + LO::EW;
   // Connectivity and face geometry of all three faces, then (FP32) the neighbours' density and energy
   // modes of all three faces, each group issued together rather than face by face. A face without a
@@ -1079,8 +1082,8 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
 // (one wavefront = one 64-element block = one case: the exits below are uniform per wavefront)
 template <class T, int P, int NQ, int NE>
 __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ U, const double* __restrict__ lt,
-                                               LimiterBufs lb, Ctl ctl, int slot, int force, int fuse_dt,
-                                               int remap) {
+                                               const T* __restrict__ tphie, LimiterBufs lb, Ctl ctl, int slot,
+                                               int force, int fuse_dt, int remap) {
   const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int e = blk * blockDim.x + threadIdx.x;
   if ((e >> 6) >= (m.nes >> 6)) return;
@@ -1093,7 +1096,7 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
     bool t = false;
     if (valid) {
       double s0, s1, s2, s3;
-      t = kxrcf<T, P, NQ, NE>(m, U, lt, e, s0, s1, s2, s3);
+      t = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, e, s0, s1, s2, s3);
       if (lb.flags) lb.flags[e] = t ? 1 : 0;
     }
     const int i = wave_append(&ctl.count[slot], t, &cc->count[slot]);
@@ -1106,7 +1109,7 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
   if (valid) {
     const double hmin = ld32(m.el_geo + (size_t)G_HMIN * m.nes, (unsigned)e), cfl = cc->cfl;
     double s0, s1, s2, s3;
-    t = kxrcf<T, P, NQ, NE>(m, U, lt, e, s0, s1, s2, s3);
+    t = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, e, s0, s1, s2, s3);
     if (lb.flags) lb.flags[e] = t ? 1 : 0;
     if (!t) {
       const double d = element_dt<1>([&](int v, int) { return v == 0 ? s0 : (v == 1 ? s1 : (v == 2 ? s2 : s3)); },
@@ -1420,8 +1423,8 @@ __global__ void k_flush(int* __restrict__ pend, const double* __restrict__ w, do
 template <class T, int P, int NQ, int NE, int STAGE>
 __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __restrict__ elems, int n, StageBufs<T> b,
                                                     const T* __restrict__ U, const double* __restrict__ lt,
-                                                    const double* __restrict__ smat, LimiterBufs lb,
-                                                    Ctl ctl, int slot, int force) {
+                                                    const T* __restrict__ tphie, const double* __restrict__ smat,
+                                                    LimiterBufs lb, Ctl ctl, int slot, int force) {
   __shared__ WenoShared<P, NQ, NE> sh;
   __shared__ double outv[4 * np_of(P)];
   constexpr int NP = np_of(P);
@@ -1434,7 +1437,7 @@ __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __rest
   if (!force && !case_runs(*cc)) return;
   if (threadIdx.x == 0) {
     double s0, s1, s2, s3;
-    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, e, s0, s1, s2, s3);
+    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, e, s0, s1, s2, s3);
     sh.flag = t ? 1 : 0;
     if (lb.flags) lb.flags[e] = t ? 1 : 0;
   }
@@ -1662,8 +1665,8 @@ class SolverImpl final : public Solver {
       const int fuse = (STAGE == 2 && slot == 2) ? 1 : 0;
       if (part == PART_DETECT) {
         const int bs = s.tune.detect_block;
-        go(s, k_kxrcf<T, P, NQ, NE>, blocks_for(s.mesh.ne, bs), bs, st, s.mesh, U, s.ltab, lb, s.ctl, slot, force,
-           fuse, s.tune.remap);
+        go(s, k_kxrcf<T, P, NQ, NE>, blocks_for(s.mesh.ne, bs), bs, st, s.mesh, U, s.ltab, (const T*)s.tab + O::PHIE,
+           lb, s.ctl, slot, force, fuse, s.tune.remap);
       } else if (part == PART_REBUILD) {
         go(s, k_weno<T, P, NQ, NE>, s.tune.weno_grid, 64, st, s.mesh, U, s.smat, lb, s.ctl, slot, fuse);
       } else if (part == PART_APPLY && slot == 3) {
@@ -1675,7 +1678,8 @@ class SolverImpl final : public Solver {
       for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
         const int n = s.level_offsets[l + 1] - s.level_offsets[l];
         k_limit_level<T, P, NQ, NE, STAGE><<<n, 64, 0, st>>>(s.mesh, s.levels + s.level_offsets[l], n, b, U, s.ltab,
-                                                             s.smat, lb, s.ctl, slot, force);
+                                                             (const T*)s.tab + O::PHIE, s.smat, lb, s.ctl, slot,
+                                                             force);
       }
     }
   }
