@@ -109,7 +109,14 @@ __device__ __forceinline__ int hllc(const T lv[4], const T rv[4], T nx, T ny, T 
   const T pl = (hcr - T(1)) * (ql3 - T(0.5) * (ql1 * ql1 + ql2 * ql2) * il);
   const T pr = (hcr - T(1)) * (qr3 - T(0.5) * (qr1 * qr1 + qr2 * qr2) * ir);
   if (ql0 < T(0) || qr0 < T(0)) return 1;
-  {
+  if constexpr (sizeof(T) == 4) {
+    // FP32 states: with gamma > 1 and rho >= 0, p < 0 iff 2 rho E < (rho u)^2 + (rho v)^2. The products of
+    // FP32 values are exact in FP64, so this is the FP64 test without its two divisions (rho = 0: a
+    // nonzero momentum fails, as -inf < 0 does; zero momentum passes, as NaN < 0 does).
+    const double l = 2.0 * (double)ql0 * (double)ql3 - ((double)ql1 * (double)ql1 + (double)ql2 * (double)ql2);
+    const double r = 2.0 * (double)qr0 * (double)qr3 - ((double)qr1 * (double)qr1 + (double)qr2 * (double)qr2);
+    if (l < 0.0 || r < 0.0) return 2;
+  } else {
     const double h = (double)hcr;
     const double pl64 = (h - 1.0) * ((double)ql3 - 0.5 * ((double)ql1 * (double)ql1 + (double)ql2 * (double)ql2) / (double)ql0);
     const double pr64 = (h - 1.0) * ((double)qr3 - 0.5 * ((double)qr1 * (double)qr1 + (double)qr2 * (double)qr2) / (double)qr0);
