@@ -175,9 +175,12 @@ def test_negative_density_reports_edge(all_meshes):
     assert ei.value.edge == bad
 
 
-def test_negative_pressure_status(all_meshes):
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_negative_pressure_status(all_meshes, precision):
+    """FP64 tests the reference's pressure; FP32-bulk tests the sign of 2 rho E - |rho u|^2 from exact FP64
+    products of its FP32 traces. Both report the same status at the same (lowest) edge."""
     mesh, order, kw = all_meshes["ogrid_p3"]
-    ref, ctx = pair(mesh, order, **kw)
+    ref, ctx = pair(mesh, order, precision=precision, **kw)
     U = flow_state(ref, seed=20, mach=0.8)
     U[77, 3, 0] = 0.0  # zero energy: negative pressure on its edges
     _, s, bad = ref.compute_residuals(U)
