@@ -898,31 +898,40 @@ __global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, 
 // (clamped to final_time), and decide whether it runs (step < final_step && t < final_time).
 // One wavefront per case slot: lanes 0 .. DT_LANES - 1 read and reset the case's dt words in parallel.
 __global__ __launch_bounds__(64) void k_step_begin(Ctl ctl, int ncases) {
+  static_assert(2 * DT_LANES == 64, "one wavefront reads the dt words of both step parities");
   const int i = blockIdx.x;
   if (i >= ncases) return;
   CaseCtl& c = ctl.cases[i];
-  if (!c.live) return;
   const int lane = threadIdx.x;
-  const int step = c.step + (c.pending ? 1 : 0);  // the step about to begin
-  unsigned long long* slot = dt_slot(ctl, i, step);
-  double v = 1.0e10;
-  if (lane < DT_LANES) {
-    v = __longlong_as_double((long long)slot[lane * DT_STRIDE]);
-    slot[lane * DT_STRIDE] = (unsigned long long)__double_as_longlong(1.0e10);
-  }
+  // The record and the dt words of both step parities are read together (lane l: parity l / 32, word
+  // l % 32); the parity of the step about to begin then selects one half, which is reset. Nothing below
+  // returns early, so no load waits behind another.
+  unsigned long long* word = dt_slot(ctl, i, lane >> 5) + (size_t)(lane & (DT_LANES - 1)) * DT_STRIDE;
+  const unsigned long long raw = *word;
+  const int live = c.live, pending = c.pending, step0 = c.step, final_step = c.final_step, cnt2 = c.count[2];
+  const double t0 = c.t, dt0 = c.dt, final_time = c.final_time;
+  const unsigned long long error = c.error;
+  const int step = step0 + (pending ? 1 : 0);  // the step about to begin
+  const bool mine = live && (lane >> 5) == (step & 1);
+  const double v = mine ? __longlong_as_double((long long)raw) : 1.0e10;
+  if (mine) *word = (unsigned long long)__double_as_longlong(1.0e10);
   const double dtm = wave_min(v);
-  if (lane != 0) return;
-  if (c.pending) {
-    c.t += c.dt;
-    c.step += 1;
+  if (lane != 0 || !live) return;
+  int st = step0;
+  double t = t0;
+  if (pending) {
+    t = t0 + dt0;
+    st = step0 + 1;
+    c.t = t;
+    c.step = st;
     c.pending = 0;
-    c.last_count = c.count[2];
+    c.last_count = cnt2;
   }
-  const bool go = c.step < c.final_step && c.t < c.final_time && c.error == ~0ull;
+  const bool go = st < final_step && t < final_time && error == ~0ull;
   c.active = go ? 1 : 0;
   if (go) {
     double dt = dtm;
-    if (c.t + dt > c.final_time) dt = c.final_time - c.t;
+    if (t + dt > final_time) dt = final_time - t;
     c.dt = dt;
     c.pending = 1;
     c.count[0] = c.count[1] = c.count[2] = 0;
