@@ -139,7 +139,8 @@ dgm_tuning from_tuning(const Tuning& u) {
 }
 static bool valid_block(int v) { return v == 64 || v == 128 || v == 256; }
 bool valid_tuning(const dgm_tuning& t) {
-  return valid_block(t.surface_block) && (t.element_waves == 1 || t.element_waves == 2 || t.element_waves == 4) &&
+  return valid_block(t.surface_block) &&
+         (t.element_waves == 0 || t.element_waves == 1 || t.element_waves == 2 || t.element_waves == 4) &&
          valid_block(t.element_block) && valid_block(t.detect_block) && (t.remap == 0 || t.remap == 1) &&
          t.weno_grid >= 64 && t.weno_grid <= 65536;
 }
@@ -379,6 +380,10 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
   S.ctl.count = count;
   S.limiter = limiter;
   S.use_mfma = solver->has_mfma() ? 1 : 0;
+  // FP32-bulk element kernel: the matrix-core one from p = 3; at p <= 2 its padded products cost more
+  // than the one-lane-per-element kernel (measured on the 128k-triangle O-grid: p = 1 18.2 vs 9.0 us,
+  // p = 2 19.5 vs 13.2 us, p = 3 23.3 vs 27.2 us per stage)
+  if (order <= 2) S.tune.element_waves = 0;
   // rebuild wavefronts: enough for a troubled fraction of 1 / 32 in one pass, 2048 at least
   S.tune.weno_grid = (int)std::min<int64_t>(65536, std::max<int64_t>(2048, cap_e / 32));
   dm.hcr = 1.4;
@@ -660,7 +665,10 @@ int autotune_steps(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned l
     };
     status = pick([](Tuning& t, int v) { t.surface_block = v; }, blocks, 3);
     if (!status) {
-      if (A->S.use_mfma && A->solver->has_mfma())
+      // the shape of the element kernel variant in use. The variant itself (matrix-core, or 0: one lane
+      // per element) is not searched: the two round FP32 sums differently, and a case must give the
+      // same state alone and in any batch whatever shapes their arenas were tuned to.
+      if (A->S.use_mfma && A->solver->has_mfma() && A->S.tune.element_waves > 0)
         status = pick([](Tuning& t, int v) { t.element_waves = v; }, waves, 3);
       else
         status = pick([](Tuning& t, int v) { t.element_block = v; }, blocks, 3);

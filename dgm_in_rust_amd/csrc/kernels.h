@@ -102,8 +102,9 @@ struct LimiterBufs {
 // Launch shapes of one arena, chosen by the autotuner (defaults are the measured best at p = 3).
 struct Tuning {
   int surface_block = 256;   // threads per block of the edge kernel (64, 128, 256)
-  int element_waves = 4;     // FP32 matrix-core element kernel: wavefronts (16 elements each) per block (1, 2, 4)
-  int element_block = 128;   // FP64 element kernel: threads per block (64, 128, 256)
+  int element_waves = 4;     // FP32 matrix-core element kernel: wavefronts (16 elements each) per block (1, 2, 4);
+                             // 0: the one-lane-per-element kernel instead (the faster one at p <= 2)
+  int element_block = 128;   // one-lane-per-element kernel (always in FP64): threads per block (64, 128, 256)
   int detect_block = 256;    // detector threads per block (64, 128, 256)
   int remap = 1;             // XCD-contiguous block order for the gather kernels
   int weno_grid = 2048;      // wavefronts of the WENO rebuild

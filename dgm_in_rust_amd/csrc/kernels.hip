@@ -1617,7 +1617,7 @@ class SolverImpl final : public Solver {
   void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force) const {
     if (s.mesh.ne <= 0) return;
     if constexpr (sizeof(T) == 4 && NP <= 16 && NQ <= 16 && 3 * NE <= 16) {
-      if (s.use_mfma) {
+      if (s.use_mfma && s.tune.element_waves > 0) {
         const int w = s.tune.element_waves;
         go(s, k_element_mfma<NP, NQ, NE, OP>, blocks_for(s.mesh.ne, 16 * w), 64 * w, st, s.mesh, b,
            (const float*)s.flux, (const float*)s.tab, s.ctl, force, s.tune.remap);

@@ -173,9 +173,10 @@ int dgm_troubled_counts(dgm_ctx* ctx, int32_t counts[4]);
 /* Lift coefficient Cl = (Fy cos a - Fx sin a) / (0.5 rho V^2 c) of the wall pressure (extension), chord c. */
 int dgm_lift_coefficient(dgm_ctx* ctx, double chord, double* cl);
 /* Launch shapes of the kernels (extension; no counterpart in the reference): threads per block of the
- * edge kernel, wavefronts per block of the FP32 matrix-core element kernel (16 elements each), threads
- * per block of the FP64 element kernel and of the detector, XCD-contiguous block order, and wavefronts
- * of the WENO rebuild. */
+ * edge kernel; FP32-bulk element kernel: wavefronts per block of the matrix-core kernel (16 elements
+ * each; 1, 2, 4), or 0 for the one-lane-per-element kernel, which the FP64 path always uses; threads per
+ * block of the one-lane-per-element kernel and of the detector; XCD-contiguous block order; and
+ * wavefronts of the WENO rebuild. */
 typedef struct {
   int32_t surface_block, element_waves, element_block, detect_block, remap, weno_grid;
 } dgm_tuning;
