@@ -141,10 +141,17 @@ def test_final_time_clamp(all_meshes):
     assert rel_err(V, Uref) <= TOL64
 
 
+@pytest.mark.parametrize("variant", ["default", "other"])
 @pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3"])
-def test_residual_fp32_bulk(all_meshes, name):
+def test_residual_fp32_bulk(all_meshes, name, variant):
+    """Both FP32-bulk element kernels (matrix-core: element_waves > 0; one lane per element: 0) against
+    the oracle, at the order where each is the default and where it is not."""
     mesh, order, kw = all_meshes[name]
     ref, ctx = pair(mesh, order, precision="fp32", **kw)
+    if variant == "other":
+        t = ctx.tuning()
+        t["element_waves"] = 4 if t["element_waves"] == 0 else 0
+        ctx.set_tuning(t)
     U = flow_state(ref, seed=17, shock=True, mach=kw["mach"])
     Rref, s, _ = ref.compute_residuals(U)
     R = SpatialDisc(ctx).residuals(U)
