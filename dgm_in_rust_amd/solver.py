@@ -221,12 +221,15 @@ class SpatialDisc:
 class TemperalDisc:
     """TemperalDisc (src/temporal_disc.rs:8-93) on the GPU (the reference's spelling is kept)."""
 
-    def __init__(self, ctx: Context, final_step: int = 20, final_time: float = 5.0):
+    def __init__(self, ctx: Context, final_step: int = 20, final_time: float = 5.0, output_dir: Optional[str] = None):
         self.ctx = ctx
         self.final_step = final_step
         self.final_time = final_time
         self.current_time = 0.0
         self.current_step = 0
+        # the reference writes <dir>/step=N.vtk at the end of every march (src/temporal_disc.rs:56); here
+        # only when a folder is given
+        self.output_dir = output_dir
 
     def compute_time_step(self, solutions: np.ndarray) -> float:
         U = np.ascontiguousarray(solutions, np.float64)
@@ -249,6 +252,9 @@ class TemperalDisc:
                                                   C.byref(st)))
         self.current_step = st.steps
         self.current_time = st.time
+        if self.output_dir is not None:
+            from .vtk import write_solution
+            write_solution(self.ctx, solutions, self.current_step, self.output_dir)
         return dict(steps=st.steps, time=st.time, last_dt=st.last_dt, troubled=st.troubled_cells_last_stage)
 
 

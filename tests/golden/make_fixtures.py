@@ -4,8 +4,11 @@ Run in the build container only (it reads the reference tree): copies the shippe
 inputfiles/airfoil.msh (Gmsh 4.1, input data) and extracts from outputfiles/mesh.vtk the points
 and triangle connectivity that the reference wrote after GmshParser + process_mesh
 (src/io/write_results.rs:9-24), i.e. the element -> vertex order, as airfoil_vtk.npz.
-The VTK cell data is all NaN and is not used.
+The VTK cell data is all NaN. The size and SHA-256 of the whole file go to airfoil_mesh_vtk.json,
+so that the product's VTK writer can be checked byte for byte against the reference's writer.
 """
+import hashlib
+import json
 import os
 import shutil
 import sys
@@ -18,7 +21,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 def main():
     shutil.copyfile(os.path.join(REF, "inputfiles", "airfoil.msh"), os.path.join(HERE, "airfoil.msh"))
-    lines = open(os.path.join(REF, "outputfiles", "mesh.vtk")).read().split("\n")
+    raw = open(os.path.join(REF, "outputfiles", "mesh.vtk"), "rb").read()
+    with open(os.path.join(HERE, "airfoil_mesh_vtk.json"), "w") as f:
+        json.dump({"file": "outputfiles/mesh.vtk", "bytes": len(raw), "sha256": hashlib.sha256(raw).hexdigest()}, f,
+                  indent=1)
+        f.write("\n")
+    lines = raw.decode().split("\n")
     i = next(k for k, l in enumerate(lines) if l.startswith("POINTS"))
     n = int(lines[i].split()[1])
     vals = []

@@ -101,19 +101,23 @@ def test_rk3_step_fp64(all_meshes, name):
     assert rel_err(V, Uref) <= TOL64, rel_err(V, Uref)
 
 
-def test_time_march_reference_config(all_meshes):
-    """airfoil.msh, p = 2, (6, 3) rules, HLLC, sequential WENO, reference CFL formula, 10 steps."""
+def test_time_march_reference_config(all_meshes, tmp_path):
+    """airfoil.msh, p = 2, (6, 3) rules, HLLC, sequential WENO, reference CFL formula, 10 steps; the
+    march writes its result file as the reference's does (step=10.vtk, cell means of the final state)."""
+    from dgm_in_rust_amd import vtk
     mesh, order, kw = all_meshes["airfoil_p2"]
     ref, ctx = pair(mesh, order, **kw)
     U0 = ref.initial_solution(1.0, 1.5, 0.8, 1.0e5)
     Uref, s, n, t = ref.time_march(U0, 10, 5.0)
     assert s == 0 and n == 10
     V = U0.copy()
-    td = TemperalDisc(ctx, final_step=10, final_time=5.0)
+    td = TemperalDisc(ctx, final_step=10, final_time=5.0, output_dir=str(tmp_path))
     st = td.time_march(SpatialDisc(ctx), ctx.zeros(), V)
     assert st["steps"] == 10
     assert abs(st["time"] - t) <= 1e-14 * t
     assert rel_err(V, Uref) <= TOL64, rel_err(V, Uref)
+    text = open(tmp_path / "step=10.vtk").read()
+    assert text == vtk.vtk_text(mesh.vxy, ctx.connectivity()["element_vertices"], V, 1.4)
 
 
 def test_time_march_ogrid_p3_jacobi(all_meshes):
