@@ -949,10 +949,12 @@ __global__ void k_reset_count(Ctl ctl, int slot) { ctl.count[slot] = 0; }
 // no inflow face gives 0/0 = NaN, which is not >= 1. The detection arithmetic runs in the bulk type
 // (FP64 in FP64 mode; FP32 in FP32-bulk mode, where only the WENO indicators and weights and the
 // positivity tests stay in FP64); the mode sums for the next time step are FP64 in both.
-template <class T, int P, int NQ, int NE>
+// after_loads() runs once every load of the element has been issued (the caller's own reads go there,
+// so that they overlap the element's instead of preceding or following them).
+template <class T, int P, int NQ, int NE, class AfterLoads>
 __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt,
                                       const T* __restrict__ tphie, int e, double& sum0, double& sum1, double& sum2,
-                                      double& sum3) {
+                                      double& sum3, AfterLoads after_loads) {
   using LO = LimOff<P, NQ, NE>;
   constexpr int NP = LO::NP;
   const int nes = m.nes;
@@ -1018,6 +1020,7 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
       }
     }
   }
+  after_loads();
   // FP32: every face is evaluated and its terms are selected in (the same operations in the same
   // order as skipping it), so no branch separates the loads above from their uses. FP64 keeps the
   // branches, which keep the neighbour loads (and their registers) to the inflow faces.
@@ -1104,39 +1107,41 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
   const int e = blk * blockDim.x + threadIdx.x;
   if ((e >> 6) >= (m.nes >> 6)) return;
   const int cs = case_of(m, m.el_case64, e);
-  if (cs < 0) return;
-  CaseCtl* cc = ctl.cases + cs;
-  if (!force && !case_runs(*cc)) return;
-  const bool valid = e < m.ne && e < cc->e_end;
-  if (!fuse_dt) {
-    bool t = false;
-    if (valid) {
-      double s0, s1, s2, s3;
-      t = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, e, s0, s1, s2, s3);
-      if (lb.flags) lb.flags[e] = t ? 1 : 0;
-    }
-    const int i = wave_append(&ctl.count[slot], t, &cc->count[slot]);
-    if (i >= 0) lb.list[i] = e;
-    if (valid && lb.pend) lb.pend[e] = i;
-    return;
-  }
-  bool t = false;
-  double dt = 1.0e10;
-  if (valid) {
-    const double hmin = ld32(m.el_geo + (size_t)G_HMIN * m.nes, (unsigned)e), cfl = cc->cfl;
-    double s0, s1, s2, s3;
-    t = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, e, s0, s1, s2, s3);
-    if (lb.flags) lb.flags[e] = t ? 1 : 0;
-    if (!t) {
+  CaseCtl* cc = ctl.cases + (cs < 0 ? 0 : cs);
+  // The case record is read here but consulted only after the element's own loads have gone out:
+  // every lane below the capacity loads in bounds (the arrays are capacity-sized, and the connectivity
+  // of unused capacity is -1 or a former case's in-range ids), and the results of a lane outside a
+  // running case's range are masked, not branched around, so the loads stay ahead of the record.
+  // (scalar loads return out of order, so a wait for any one waits for all: the record is read just
+  // after the element's loads are issued, and nothing before them branches on a kernel argument)
+  const double hmin = ld32(m.el_geo + (size_t)G_HMIN * m.nes, (unsigned)e);
+  double s0, s1, s2, s3;
+  int c_active, e_end, c_step;
+  unsigned long long c_error;
+  double cfl;
+  const bool det = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, e, s0, s1, s2, s3, [&]() {
+    c_active = cc->active;
+    e_end = cc->e_end;
+    c_step = cc->step;
+    c_error = cc->error;
+    cfl = cc->cfl;
+  });
+  const bool runs = (cs >= 0) & ((force != 0) | ((c_active != 0) & (c_error == ~0ull)));  // case_runs()
+  const bool valid = runs & (e < m.ne) & (e < e_end);
+  const bool t = det & valid;
+  if (valid && lb.flags) lb.flags[e] = t ? 1 : 0;
+  const int i = wave_append(&ctl.count[slot], t, &cc->count[slot]);
+  if (i >= 0) lb.list[i] = e;
+  if (valid && lb.pend) lb.pend[e] = i;
+  if (fuse_dt && runs) {
+    double dt = 1.0e10;
+    if (valid && !t) {
       const double d = element_dt<1>([&](int v, int) { return v == 0 ? s0 : (v == 1 ? s1 : (v == 2 ? s2 : s3)); },
                                      m.hcr, cfl, hmin);
       if (d < dt) dt = d;
     }
+    wave_min_dt(dt, dt_slot(ctl, cs, c_step + 1), e >> 6);
   }
-  const int i = wave_append(&ctl.count[slot], t, &cc->count[slot]);
-  if (i >= 0) lb.list[i] = e;
-  if (valid && lb.pend) lb.pend[e] = i;
-  wave_min_dt(dt, dt_slot(ctl, cs, cc->step + 1), e >> 6);
 }
 
 // Smoothness matrix of one element: beta(P) = sum_{l=1..p} J^(l-1) sum_{l1=0..l} sum_q (J/2) w_q
@@ -1381,9 +1386,14 @@ __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U,
                                              LimiterBufs lb, Ctl ctl, int slot, int fuse_dt) {
   __shared__ WenoShared<P, NQ, NE> sh;
   constexpr int NP = np_of(P);
+  // the block's first list entry is read in the same trip as the count (at a clamped index: the list
+  // holds nes entries, and an entry at or past the count is stale and left unused)
+  int s = blockIdx.x;
   const int n = ctl.count[slot];
-  for (int s = blockIdx.x; s < n; s += gridDim.x) {
-    const int e = lb.list[s];
+  int e = lb.list[s < m.nes ? s : m.nes - 1];
+  asm volatile("" : "+s"(e));  // keeps the load ahead of the loop test (it would be sunk below it)
+  for (; s < n; s += gridDim.x) {
+    if (s != (int)blockIdx.x) e = lb.list[s];
     const int cs = case_of(m, m.el_case64, e);
     CaseCtl* cc = ctl.cases + cs;
     double* out = lb.w + (size_t)s * 4 * NP;
@@ -1453,7 +1463,7 @@ __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __rest
   if (!force && !case_runs(*cc)) return;
   if (threadIdx.x == 0) {
     double s0, s1, s2, s3;
-    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, e, s0, s1, s2, s3);
+    const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, e, s0, s1, s2, s3, []() {});
     sh.flag = t ? 1 : 0;
     if (lb.flags) lb.flags[e] = t ? 1 : 0;
   }
