@@ -688,6 +688,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const float ex = ld32(eg + (size_t)G_EX * nes, eu), ey = ld32(eg + (size_t)G_EY * nes, eu);
   const float j2 = ld32(eg + (size_t)G_2J * nes, eu);
   const int pend_slot = (OP != OP_RESID && b.pend) ? ld32(b.pend, eu) : -1;
+  // the RK operands of the lane's entries go out with the rest, so the epilogue waits on no load of its
+  // own; a pending write-back below rewrites d of its element and patches d0 with the value it stores
+  float d0[4][4];
+  double un0[4][4];
+  if (OP == OP_S1 || OP == OP_S2) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 4 * g + r;
+        const unsigned ix = (unsigned)((k < NP ? k : NP - 1) * nes + es);
+        d0[v][r] = ld32(b.d + (size_t)v * NP * nes, ix);
+        if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
+      }
+  }
   const bool runs = (force != 0) | ((c_active != 0) & (c_error == ~0ull));  // case_runs()
   if (!runs) {
     const int ew = wave * 16 + (lane & 15);
@@ -726,6 +741,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (k < NP) {
             us[v][s] = (float)w[v * NP + k];
             store_limited<float, NP, PREV>(b, nes, e, v * NP + k, w[v * NP + k]);
+            // the d just stored (store_limited, stages 0 and 1: d = w - u^n)
+            if constexpr (OP == OP_S1 || OP == OP_S2)
+              d0[v][s] = (float)(w[v * NP + k] - b.un[(size_t)(v * NP + k) * nes + e]);
           }
         }
     }
@@ -762,20 +780,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       acc[v] = mfma4(aDxi[s], ft[v][s], acc[v]);
       acc[v] = mfma4(aDeta[s], gt[v][s], acc[v]);
     }
-  }
-  // the RK operands of the lane's entries, read after any pending write-back above and before any store
-  float d0[4][4];
-  double un0[4][4];
-  if (OP == OP_S1 || OP == OP_S2) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = 4 * g + r;
-        const unsigned ix = (unsigned)((k < NP ? k : NP - 1) * nes + es);
-        d0[v][r] = ld32(b.d + (size_t)v * NP * nes, ix);
-        if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
-      }
   }
   // lift operands: K index = face * NE + point
   float aLift[4];
