@@ -88,6 +88,7 @@ CaseCtl fresh_case(double cfl) {
   h.final_time = 1.0e300;
   h.cfl = cfl;
   h.error = ~0ull;
+  h.error_s0 = ~0ull;
   h.final_step = 0x7fffffff;
   h.active = 1;
   h.live = 1;

@@ -108,10 +108,15 @@ void enqueue_step(dgm_batch* b) {
   if (b->prm.limiter != DGM_LIMITER_JACOBI)
     for (int g = 0; g < 4; ++g)
       if (b->arena[g] && b->arena[g]->extent_elements() > 0) b->arena[g]->solver->time_step(b->arena[g]->S, b->stream, 0);
-  launch_step_begin(shared_ctl(b), b->prm.max_cases, b->stream);
+  // the step control of every case slot runs inside the first arena's stage-0 edge launch
+  int nctl = b->prm.max_cases;
   for (int s = 0; s < 3; ++s)
     for (int g = 0; g < 4; ++g)
-      if (b->arena[g] && b->arena[g]->extent_elements() > 0) b->arena[g]->solver->stage(b->arena[g]->S, s, b->stream);
+      if (b->arena[g] && b->arena[g]->extent_elements() > 0) {
+        b->arena[g]->solver->stage(b->arena[g]->S, s, b->stream, s == 0 ? nctl : 0);
+        if (s == 0) nctl = 0;
+      }
+  if (nctl > 0) launch_step_begin(shared_ctl(b), nctl, b->stream);  // no arena holds elements
 }
 
 int ensure_graph(dgm_batch* b) {
@@ -292,7 +297,7 @@ int dgm_batch_remove(dgm_batch* b, int32_t id) {
   BatchCase& bc = b->cases[id];
   CaseCtl h;
   std::memset(&h, 0, sizeof(h));
-  h.error = ~0ull;
+  h.error = h.error_s0 = ~0ull;
   if (int s = put_case(b, id, h)) return s;
   Arena& A = *b->arena[bc.order - 1];
   std::string err;

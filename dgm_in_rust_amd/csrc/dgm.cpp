@@ -125,10 +125,10 @@ int stage_out(dgm_ctx* c, const double* soa, double* U) {
 // stand-alone reduction at the start of each step.
 bool fused_dt(const dgm_ctx* c) { return c->prm.limiter == DGM_LIMITER_JACOBI; }
 
+// the step control runs inside the stage-0 edge launch (one extra wavefront, beside the edges)
 void enqueue_step(dgm_ctx* c) {
   if (!fused_dt(c)) solver(c).time_step(dev(c), c->stream, 0);
-  launch_step_begin(dev(c).ctl, 1, c->stream);
-  for (int s = 0; s < 3; ++s) solver(c).stage(dev(c), s, c->stream);
+  for (int s = 0; s < 3; ++s) solver(c).stage(dev(c), s, c->stream, s == 0 ? 1 : 0);
 }
 
 // fresh control block for a resident run, with the first step's time step already reduced
@@ -535,8 +535,8 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   if (!c || !c->stream || !out || n < 1) return DGM_BAD_ARGUMENT;
   std::memset(out, 0, sizeof(*out));
   constexpr int NP_ = Solver::N_PARTS;
-  // per part of each stage: a start and a stop event; then 4 stream events: step start, after the
-  // time-step reduction, after the step control, step end
+  // per part of each stage: a start and a stop event; then stream events: step start, after the
+  // time-step reduction, step end (ctl_ev[2] is unused)
   std::vector<hipEvent_t> ev(2 * 3 * NP_ + 4);
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
   hipEvent_t* ctl_ev = &ev[2 * 3 * NP_];
@@ -554,25 +554,25 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   // own start and end; the element-order sweep (several launches) is bracketed by stream events
   auto single = [lim](int part) { return part <= Solver::PART_ELEMENT || lim == DGM_LIMITER_JACOBI; };
   double acc[NP_] = {0, 0, 0, 0, 0};
-  double t_dt = 0.0, t_ctl = 0.0, t_step = 0.0;
+  double t_dt = 0.0, t_step = 0.0;
   for (int it = 0; it < n; ++it) {
     HIPCHK(c, hipEventRecord(ctl_ev[0], c->stream));
     if (!fused_dt(c)) solver(c).time_step(S, c->stream, 0);
     HIPCHK(c, hipEventRecord(ctl_ev[1], c->stream));
-    launch_step_begin(S.ctl, 1, c->stream);
-    HIPCHK(c, hipEventRecord(ctl_ev[2], c->stream));
+    // the step control runs inside the stage-0 edge launch, as in a march: no launch of its own
     for (int st = 0; st < 3; ++st)
       for (int part = 0; part < NP_; ++part) {
         if (!launches(part)) continue;
         hipEvent_t* pe = &ev[2 * (st * NP_ + part)];
+        const int nctl = (st == 0 && part == Solver::PART_SURFACE) ? 1 : 0;
         if (single(part)) {
           S.prof_ev[0] = pe[0];
           S.prof_ev[1] = pe[1];
-          solver(c).stage_part(S, st, part, c->stream);
+          solver(c).stage_part(S, st, part, c->stream, nctl);
           S.prof_ev[0] = S.prof_ev[1] = nullptr;
         } else {
           HIPCHK(c, hipEventRecord(pe[0], c->stream));
-          solver(c).stage_part(S, st, part, c->stream);
+          solver(c).stage_part(S, st, part, c->stream, nctl);
           HIPCHK(c, hipEventRecord(pe[1], c->stream));
         }
       }
@@ -584,8 +584,6 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
       HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[0], ctl_ev[1]));
       t_dt += ms;
     }
-    HIPCHK(c, hipEventElapsedTime(&ms, ctl_ev[1], ctl_ev[2]));
-    t_ctl += ms;
     for (int st = 0; st < 3; ++st)
       for (int part = 0; part < NP_; ++part) {
         if (!launches(part)) continue;
@@ -598,7 +596,7 @@ int dgm_profile_steps(dgm_ctx* c, int32_t n, dgm_kernel_times* out) {
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   out->time_step_ms = t_dt / n;
-  out->control_ms = t_ctl / n;
+  out->control_ms = 0.0;
   out->surface_ms = acc[0] / (3.0 * n);
   out->element_ms = acc[1] / (3.0 * n);
   out->detect_ms = acc[2] / (3.0 * n);

@@ -57,7 +57,12 @@ struct CaseCtl {
   int group;                        // arena (order) of the case
   int count[4];                     // troubled cells per stage of the running step (slot 3: stand-alone limiter)
   int last_count;                   // troubled cells of the last stage of the previous step
-  int pad[7];
+  int pad0;
+  // first failing edge of the stage-0 edge pass of the running step (same coding as error): that pass
+  // runs beside the step control, which reads error, so it records here and the stage-0 element pass
+  // moves it into error when the case runs the step
+  unsigned long long error_s0;
+  int pad[4];
 };
 static_assert(sizeof(CaseCtl) == 128, "CaseCtl is one 128-byte record");
 

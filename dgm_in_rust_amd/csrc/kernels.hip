@@ -349,13 +349,28 @@ __device__ __forceinline__ void edge_traces(F coef, const T* sphi, int lic, bool
       for (int q = 0; q < NE; ++q) tr[q][v] += c * sphi[(lic * NE + (rev ? NE - 1 - q : q)) * NP + k];
     }
 }
+__device__ __forceinline__ void step_control(const Ctl& ctl, int i, int lane);
+
+// Stage 0 of a step (s0) runs beside the step control of that step: with nctl > 0 the last blocks of
+// the grid run it for case slots 0 .. nctl - 1, one wavefront per slot. The edges of stage 0 then read
+// the case's active flag as it stands (the step control may be clearing it: either value is safe, since
+// a case computed in vain records its errors into error_s0, which the stage-0 element pass takes over
+// only when the case runs the step) and never write error.
 template <class T, int NP, int NQ, int NE>
 __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict__ U, T* __restrict__ G,
                                                  const T* __restrict__ tab, Ctl ctl, int force, int remap,
-                                                 const int* __restrict__ pend, const double* __restrict__ pw) {
+                                                 const int* __restrict__ pend, const double* __restrict__ pw, int s0,
+                                                 int nctl) {
   using O = TabOff<NP, NQ, NE>;
   __shared__ T sphi[3 * NE * NP];
-  const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int wpb = (int)(blockDim.x >> 6);
+  const int nblk = (int)gridDim.x - (nctl + wpb - 1) / wpb;  // blocks of edges; the rest: step control
+  if ((int)blockIdx.x >= nblk) {  // uniform per block, before any barrier
+    const int i = ((int)blockIdx.x - nblk) * wpb + (int)(threadIdx.x >> 6);
+    if (i < nctl) step_control(ctl, i, (int)(threadIdx.x & 63));
+    return;
+  }
+  const int blk = remap ? xcd_block(blockIdx.x, nblk) : (int)blockIdx.x;
   const int ed = blk * blockDim.x + threadIdx.x;
   const int nes = m.nes, est = m.ed_stride;
   // the edge record (read at a clamped index) goes out before the table staging and its barrier
@@ -367,7 +382,8 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
   // and so does the case record (of case 0 for edges outside any case, which exit below)
   const int cs = case_of(m, m.ed_case64, ee);
   CaseCtl* cc = ctl.cases + (cs < 0 ? 0 : cs);
-  const int c_active = cc->active, c_ed_end = cc->ed_end;
+  const int c_active = s0 ? __hip_atomic_load(&cc->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : cc->active;
+  const int c_ed_end = cc->ed_end;
   const unsigned long long c_error = cc->error;
   for (int i = threadIdx.x; i < 3 * NE * NP; i += blockDim.x) sphi[i] = tab[O::PHIE + i];
   __syncthreads();
@@ -401,7 +417,12 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
 #pragma unroll
       for (int v = 0; v < 4; ++v) g[q * 4 + v] = f[v] * hl;
     }
-    if (err) record_error(cc, ed - cc->ed_off, err);
+    if (err) {
+      if (s0)
+        atomicMin(&cc->error_s0, ((unsigned long long)(unsigned)(ed - cc->ed_off) << 8) | (unsigned long long)err);
+      else
+        record_error(cc, ed - cc->ed_off, err);
+    }
   } else if (bc == BC_WALL) {
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
@@ -438,7 +459,11 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   const int cs = case_of(m, m.el_case64, e);
   if (cs < 0) return;
   const CaseCtl& cc = ctl.cases[cs];
-  if (!force && !case_runs(cc)) {
+  // stage 0: an error of this step's edge pass (error_s0) stops the case here and becomes its error
+  const unsigned long long es0 = OP == OP_S0 ? cc.error_s0 : ~0ull;
+  if (!force && !(case_runs(cc) && es0 == ~0ull)) {
+    if (OP == OP_S0 && es0 != ~0ull && cc.active && cc.error == ~0ull && (threadIdx.x & 63) == 0)
+      atomicMin(&ctl.cases[cs].error, es0);
     if (OP == OP_S0 && b.pend && cc.live && e < cc.e_end) write_back_pending<T, NP>(b, e, m.nes);
     return;
   }
@@ -643,6 +668,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // them alone, not on the loads issued after them
   const int c_active = cc.active, c_live = cc.live, e_end = cc.e_end;
   const unsigned long long c_error = cc.error;
+  // stage 0: an error of this step's edge pass (error_s0) stops the case here and becomes its error
+  const unsigned long long c_es0 = OP == OP_S0 ? cc.error_s0 : ~0ull;
   const double dt = cc.dt;
   const int c = lane & 15, g = lane >> 4;
   const int e = wave * 16 + c;
@@ -703,8 +730,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
       }
   }
-  const bool runs = (force != 0) | ((c_active != 0) & (c_error == ~0ull));  // case_runs()
+  const bool runs = (force != 0) | ((c_active != 0) & (c_error == ~0ull) & (c_es0 == ~0ull));  // case_runs()
   if (!runs) {
+    if (OP == OP_S0 && c_es0 != ~0ull && c_active && c_error == ~0ull && lane == 0)
+      atomicMin(&ctl.cases[cs].error, c_es0);
     const int ew = wave * 16 + (lane & 15);
     if (OP == OP_S0 && b.pend && c_live && lane < 16 && ew < m.ne && ew < e_end)
       write_back_pending<float, NP>(b, ew, m.nes);
@@ -900,13 +929,12 @@ __global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, 
 
 // Step control (src/temporal_disc.rs:22-34): count the step that just ran, take its successor's dt
 // (clamped to final_time), and decide whether it runs (step < final_step && t < final_time).
-// One wavefront per case slot: lanes 0 .. DT_LANES - 1 read and reset the case's dt words in parallel.
-__global__ __launch_bounds__(64) void k_step_begin(Ctl ctl, int ncases) {
+// One wavefront per case slot i: lanes 0 .. DT_LANES - 1 read and reset the case's dt words in parallel.
+// It reads error but never error_s0, so the stage-0 edge pass, which records into error_s0, can run in
+// the same launch (k_surface) without changing what it decides.
+__device__ __forceinline__ void step_control(const Ctl& ctl, int i, int lane) {
   static_assert(2 * DT_LANES == 64, "one wavefront reads the dt words of both step parities");
-  const int i = blockIdx.x;
-  if (i >= ncases) return;
   CaseCtl& c = ctl.cases[i];
-  const int lane = threadIdx.x;
   // The record and the dt words of both step parities are read together (lane l: parity l / 32, word
   // l % 32); the parity of the step about to begin then selects one half, which is reset. Nothing below
   // returns early, so no load waits behind another.
@@ -940,6 +968,10 @@ __global__ __launch_bounds__(64) void k_step_begin(Ctl ctl, int ncases) {
     c.pending = 1;
     c.count[0] = c.count[1] = c.count[2] = 0;
   }
+}
+
+__global__ __launch_bounds__(64) void k_step_begin(Ctl ctl, int ncases) {
+  if ((int)blockIdx.x < ncases) step_control(ctl, blockIdx.x, threadIdx.x);
 }
 
 __global__ void k_reset_count(Ctl ctl, int slot) { ctl.count[slot] = 0; }
@@ -1621,11 +1653,16 @@ class SolverImpl final : public Solver {
     return stage == 2 ? (const T*)s.sn : (const T*)s.s;
   }
 
-  void surface(const DeviceState& s, const StageBufs<T>& b, hipStream_t st, int force) const {
-    if (s.mesh.nedge <= 0) return;
-    const int bs = s.tune.surface_block;
-    go(s, k_surface<T, NP, NQ, NE>, blocks_for(s.mesh.nedge, bs), bs, st, s.mesh, b.in, (T*)s.flux, (const T*)s.tab,
-       s.ctl, force, s.tune.remap, b.pend, b.pw);
+  // s0: the stage-0 edge pass of a march (errors into error_s0); nctl > 0: it also runs the step control
+  void surface(const DeviceState& s, const StageBufs<T>& b, hipStream_t st, int force, int s0 = 0, int nctl = 0) const {
+    if (s.mesh.nedge <= 0) {
+      if (nctl > 0) launch_step_begin(s.ctl, nctl, st);
+      return;
+    }
+    const int bs = s.tune.surface_block, wpb = bs / 64;
+    const int extra = nctl > 0 ? (nctl + wpb - 1) / wpb : 0;
+    go(s, k_surface<T, NP, NQ, NE>, blocks_for(s.mesh.nedge, bs) + extra, bs, st, s.mesh, b.in, (T*)s.flux,
+       (const T*)s.tab, s.ctl, force, s.tune.remap, b.pend, b.pw, s0, nctl);
   }
   template <int OP>
   void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force) const {
@@ -1657,16 +1694,16 @@ class SolverImpl final : public Solver {
     for (int part = PART_DETECT; part < N_PARTS; ++part) limiter_part<2>(s, st, 1, 3, part);
   }
 
-  void stage(const DeviceState& s, int stage_idx, hipStream_t st) const override {
-    for (int part = 0; part < N_PARTS; ++part) stage_part(s, stage_idx, part, st);
+  void stage(const DeviceState& s, int stage_idx, hipStream_t st, int nctl) const override {
+    for (int part = 0; part < N_PARTS; ++part) stage_part(s, stage_idx, part, st, nctl);
   }
 
   // one launch (or, for the element-order sweep, one launch sequence) of a stage
-  void stage_part(const DeviceState& s, int stage_idx, int part, hipStream_t st) const override {
+  void stage_part(const DeviceState& s, int stage_idx, int part, hipStream_t st, int nctl) const override {
     StageBufs<T> b = bufs(s);
     b.in = stage_input(s, stage_idx);
     if (part == PART_SURFACE) {
-      surface(s, b, st, 0);
+      surface(s, b, st, 0, stage_idx == 0 ? 1 : 0, stage_idx == 0 ? nctl : 0);
     } else if (part == PART_ELEMENT) {
       if (stage_idx == 0)
         element<OP_S0>(s, b, st, 0);

@@ -49,12 +49,14 @@ class Solver {
   virtual void residual(const DeviceState& s, hipStream_t st) const = 0;
   // SpatialDisc::apply_limiter on the resident u^n (always runs)
   virtual void limiter(const DeviceState& s, hipStream_t st) const = 0;
-  // RK stage 0, 1, 2 (surface + element + limiter), each case gated by its CaseCtl::active
-  virtual void stage(const DeviceState& s, int stage_idx, hipStream_t st) const = 0;
+  // RK stage 0, 1, 2 (surface + element + limiter), each case gated by its CaseCtl::active. With
+  // nctl > 0 (stage 0), the stage's edge launch also runs the step control of case slots 0 .. nctl - 1,
+  // which then must not be launched on its own.
+  virtual void stage(const DeviceState& s, int stage_idx, hipStream_t st, int nctl = 0) const = 0;
   // the parts of a stage, in order: surface flux, element update, detector (or the whole
   // element-order sweep), WENO rebuild, apply
   enum { PART_SURFACE = 0, PART_ELEMENT = 1, PART_DETECT = 2, PART_REBUILD = 3, PART_APPLY = 4, N_PARTS = 5 };
-  virtual void stage_part(const DeviceState& s, int stage_idx, int part, hipStream_t st) const = 0;
+  virtual void stage_part(const DeviceState& s, int stage_idx, int part, hipStream_t st, int nctl = 0) const = 0;
   // per-element smoothness matrices of the WENO indicator (set-up), elements e0 .. e0 + n - 1
   virtual int smoothness_len() const = 0;
   virtual void build_smoothness(const DeviceState& s, int e0, int n, hipStream_t st) const = 0;

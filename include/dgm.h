@@ -161,7 +161,9 @@ int dgm_device_clock(dgm_ctx* ctx, double* time, int32_t* steps, double* last_dt
 /* Per-kernel timing: runs n whole steps on the resident state (eagerly, not from the graph) on the
  * context stream and reports the mean milliseconds per launch of each stage part. A part that is one
  * kernel launch is timed by events its dispatch records at its own start and end (hipExtLaunchKernel);
- * a part of several launches (the element-order limiter sweep) by stream events around it. */
+ * a part of several launches (the element-order limiter sweep) by stream events around it. The step
+ * control runs inside the stage-0 edge launch, as in a march: surface_ms includes it, and control_ms
+ * (kept for the layout) is 0. */
 typedef struct {
   double surface_ms, element_ms, detect_ms, rebuild_ms, apply_ms, time_step_ms, control_ms;
   double step_ms;  /* mean wall time of one step between its first and last event */

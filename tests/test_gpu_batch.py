@@ -123,6 +123,9 @@ def test_per_case_error_is_isolated():
     b.synchronize()
     info = b.info(4)
     assert info[x]["status"] == 2 and info[x]["active"] == 0
+    # the failure is found by the first step's edge pass, which runs beside that step's control: the
+    # step had begun, so it is counted, and no later step runs
+    assert info[x]["steps"] == 1
     assert info[g]["status"] == 0 and info[g]["steps"] == 12
     ref, _, _ = alone(good, 2, 0.6, 1.0, 12, "fp64")
     assert np.array_equal(b.state(g), ref)

@@ -199,3 +199,21 @@ def test_negative_pressure_status(all_meshes, precision):
     with pytest.raises(DgmError) as ei:
         SpatialDisc(ctx).residuals(U)
     assert ei.value.status == 2 and ei.value.edge == bad
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_march_stage0_failure_status(all_meshes, precision):
+    """A march whose first edge pass fails (that pass runs beside the step control and records into a
+    word of its own, which the element pass takes over) stops with the status and lowest failing edge
+    of the stand-alone residual, and its state is the one it started from."""
+    mesh, order, kw = all_meshes["ogrid_p3"]
+    ref, ctx = pair(mesh, order, precision=precision, limiter="jacobi", **kw)
+    U = flow_state(ref, seed=20, mach=0.8)
+    U[77, 3, 0] = 0.0
+    _, s, bad = ref.compute_residuals(U)
+    assert s == 2
+    V = U.copy()
+    with pytest.raises(DgmError) as ei:
+        TemperalDisc(ctx, final_step=5, final_time=1.0e300).time_march(SpatialDisc(ctx), ctx.zeros(), V)
+    assert ei.value.status == 2 and ei.value.edge == bad
+    assert np.array_equal(V, U)
