@@ -1274,49 +1274,60 @@ struct WenoShared {
   double RES[3][4][NP];
   double area[3];
   double fin[4 * NP];  // the rebuilt polynomial, for the time step of the fused reduction
-  int nb[3];
-  int edge[3];
   int flag;
 };
 
-template <class T, int P, int NQ, int NE>
+// after_loads() runs once the element's loads are issued, before the first barrier (the caller's own
+// reads go there, so that they overlap the rebuild's instead of preceding them).
+template <class T, int P, int NQ, int NE, class AfterLoads>
 __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ smat, double hcr, int e,
-                         WenoShared<P, NQ, NE>& sh, double* __restrict__ out /* [4][NP] */) {
+                         WenoShared<P, NQ, NE>& sh, double* __restrict__ out /* [4][NP] */, AfterLoads after_loads) {
   constexpr int NP = np_of(P);
   constexpr int NPK = NP * (NP + 1) / 2;
   const int lane = threadIdx.x;
   const int nes = m.nes;
-  // neighbour and edge ids of the three faces, read once and broadcast over the wavefront; every
-  // other load depends on them only, so all are issued together before one barrier
-  int nbl = -1, edl = 0;
-  if (lane < 3) {
-    nbl = m.el_nbr[lane * nes + e];
-    edl = m.el_edge[lane * nes + e] >> 1;
+  // Two memory trips. First what depends on the element only: its neighbour and edge ids (the same for
+  // every lane: e is uniform over the wavefront), its smoothness matrix. Then, all together, what depends
+  // on those ids: the candidates' modes, the neighbours' areas and the face normals. Every load reads a
+  // valid index (the element itself where a face has no neighbour) and the unused values are masked
+  // afterwards, so no branch holds a load back.
+  const int nbr[3] = {m.el_nbr[e], m.el_nbr[nes + e], m.el_nbr[2 * nes + e]};
+  const int edr[3] = {m.el_edge[e] >> 1, m.el_edge[nes + e] >> 1, m.el_edge[2 * nes + e] >> 1};
+  constexpr int NBQ = (NPK + 63) / 64;
+  double bq[NBQ];
+#pragma unroll
+  for (int it = 0; it < NBQ; ++it) {
+    const int idx = lane + 64 * it;
+    bq[it] = smat[(size_t)e * NPK + (idx < NPK ? idx : 0)];
   }
-  const int nbr[3] = {__shfl(nbl, 0, 64), __shfl(nbl, 1, 64), __shfl(nbl, 2, 64)};
-  const int edr[3] = {__shfl(edl, 0, 64), __shfl(edl, 1, 64), __shfl(edl, 2, 64)};
-  if (lane < 3) {
-    sh.nb[lane] = nbl;
-    sh.edge[lane] = edl;
-    sh.area[lane] = nbl >= 0 ? m.el_geo[G_J * nes + nbl] * 0.5 : 0.0;
-  }
-  for (int idx = lane; idx < NPK; idx += 64) sh.B[idx] = smat[(size_t)e * NPK + idx];
-  for (int idx = lane; idx < 16 * NP; idx += 64) {
+  // candidate j, variable v, mode k at idx = (j * 4 + v) * NP + k, three per lane
+  constexpr int NSRC = 16 * NP, NIT = (NSRC + 63) / 64;
+  double sv[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx0 = lane + 64 * it, idx = idx0 < NSRC ? idx0 : 0;
     const int j = idx / (4 * NP), v = (idx / NP) % 4, k = idx % NP;
     const int nj = j > 0 ? nbr[j - 1] : e;
-    double val = 0.0;
-    if (j == 0 || k == 0)
-      val = (double)U[(size_t)(v * NP + k) * nes + e];
-    else if (nj >= 0)
-      val = (double)U[(size_t)(v * NP + k) * nes + nj];
-    if (j > 0 && nj < 0) val = 0.0;  // no neighbour across the face: no candidate
-    sh.src[j][v][k] = val;
+    // mode 0 of every candidate is the element's own (weno.rs:148-155)
+    const int src = (j == 0 || k == 0 || nj < 0) ? e : nj;
+    const double val = (double)U[(size_t)(v * NP + k) * nes + src];
+    sv[it] = (j > 0 && nj < 0) ? 0.0 : val;  // no neighbour across the face: no candidate
   }
-  // face normal of the characteristic lanes (lane 16 i + ...: face i)
-  double fnx = 0.0, fny = 0.0;
-  if (lane < 48 && nbr[lane >> 4] >= 0) {
-    fnx = m.ed_geo[edr[lane >> 4]];
-    fny = m.ed_geo[m.ed_stride + edr[lane >> 4]];
+  const int nbi = lane < 3 ? nbr[lane] : -1;
+  const double ar = m.el_geo[G_J * nes + (nbi >= 0 ? nbi : e)];
+  // face normal of the characteristic lanes (lane 16 i + ...: face i), used only where face i has a
+  // neighbour
+  const int fe = edr[lane < 48 ? lane >> 4 : 0];
+  const double fnx = m.ed_geo[fe], fny = m.ed_geo[m.ed_stride + fe];
+  after_loads();
+  if (lane < 3) sh.area[lane] = nbi >= 0 ? ar * 0.5 : 0.0;
+#pragma unroll
+  for (int it = 0; it < NBQ; ++it)
+    if (lane + 64 * it < NPK) sh.B[lane + 64 * it] = bq[it];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int idx = lane + 64 * it;
+    if (idx < NSRC) sh.src[idx / (4 * NP)][(idx / NP) % 4][idx % NP] = sv[it];
   }
   // linear weights, weno.rs:93-110 (a boundary face i zeroes weights[i], then weights[count] is set)
   const bool b0 = nbr[0] < 0, b1 = nbr[1] < 0, b2 = nbr[2] < 0;
@@ -1430,18 +1441,17 @@ __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U,
   asm volatile("" : "+s"(e));  // keeps the load ahead of the loop test (it would be sunk below it)
   for (; s < n; s += gridDim.x) {
     if (s != (int)blockIdx.x) e = lb.list[s];
-    const int cs = case_of(m, m.el_case64, e);
-    CaseCtl* cc = ctl.cases + cs;
     double* out = lb.w + (size_t)s * 4 * NP;
-    // what the fused time step needs is read before the rebuild, whose barriers keep it there
+    // the case and what the fused time step needs are read once the rebuild's own loads are out
+    int cs = 0, step = 0;
     double hmin = 0.0, cfl = 0.0;
-    int step = 0;
-    if (fuse_dt) {
+    const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, e, sh, out, [&]() {
+      cs = case_of(m, m.el_case64, e);
       hmin = m.el_geo[G_HMIN * m.nes + e];
-      cfl = cc->cfl;
-      step = cc->step;
-    }
-    const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, e, sh, out);
+      cfl = ctl.cases[cs].cfl;
+      step = ctl.cases[cs].step;
+    });
+    CaseCtl* cc = ctl.cases + cs;
     if (st && threadIdx.x == 0) atomicMin(&cc->error, (0xFFFFFFull << 8) | (unsigned long long)st);
     __syncthreads();
     if (fuse_dt && !st && threadIdx.x == 0) {
@@ -1505,7 +1515,7 @@ __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __rest
   }
   __syncthreads();
   if (!sh.flag) return;
-  const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, e, sh, outv);
+  const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, e, sh, outv, []() {});
   __syncthreads();
   if (st) {
     if (threadIdx.x == 0)
