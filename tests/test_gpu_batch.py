@@ -108,17 +108,19 @@ def test_capacity_beyond_offset_range_is_rejected():
     b.close()
 
 
-def test_per_case_error_is_isolated():
+@pytest.mark.parametrize("good_order,bad_order", [(2, 2), (1, 2), (2, 1)])
+def test_per_case_error_is_isolated(good_order, bad_order):
     """A case whose state fails the positivity check stops with its own status; the others go on
-    and match their stand-alone runs."""
+    and match their stand-alone runs. The failing case sits in the same arena as the good one, in a
+    later one, or in the first one (whose stage-0 edge launch also runs every case's step control)."""
     b = make_batch("fp64", cap=8000, max_cases=4)
     good = meshgen.naca0012_ogrid(40, 12, seed=31)
     bad = meshgen.naca0012_ogrid(32, 8, seed=32)
-    g = b.add(good, 2, 0.6, 1.0, final_step=12)
-    ctx = Context(bad, order=2, mach=0.5, aoa_deg=0.0, limiter="jacobi")
+    g = b.add(good, good_order, 0.6, 1.0, final_step=12)
+    ctx = Context(bad, order=bad_order, mach=0.5, aoa_deg=0.0, limiter="jacobi")
     Ub = ctx.initial_solution(1.0, 0.0, 0.5, 1.0e5)
     Ub[5, 3, 0] = -1.0e5   # negative energy -> negative pressure at element 5
-    x = b.add(bad, 2, 0.5, 0.0, U0=Ub, final_step=12)
+    x = b.add(bad, bad_order, 0.5, 0.0, U0=Ub, final_step=12)
     b.steps_async(12)
     b.synchronize()
     info = b.info(4)
@@ -127,7 +129,7 @@ def test_per_case_error_is_isolated():
     # step had begun, so it is counted, and no later step runs
     assert info[x]["steps"] == 1
     assert info[g]["status"] == 0 and info[g]["steps"] == 12
-    ref, _, _ = alone(good, 2, 0.6, 1.0, 12, "fp64")
+    ref, _, _ = alone(good, good_order, 0.6, 1.0, 12, "fp64")
     assert np.array_equal(b.state(g), ref)
     b.close()
 
