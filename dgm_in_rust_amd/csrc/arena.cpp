@@ -247,9 +247,9 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
   const bool f32 = solver->fp32();
 
   // bulk tables: PHIV, DXI, DETA [nq][np] (derivatives pre-divided by the mass, times the weight);
-  // PHIE, LIFT [3][ne][np]
+  // PHIE, LIFT [3][ne][np]; KXW [3][np] (the detector's neighbour-side jump weights)
   std::vector<double> tab(solver->table_len(), 0.0);
-  if ((int)tab.size() != 3 * nqq * np + 6 * nep * np) {
+  if ((int)tab.size() != 3 * nqq * np + 6 * nep * np + 3 * np) {
     err = "table size mismatch";
     return DGM_BAD_CONFIG;
   }
@@ -267,6 +267,12 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
         tab[3 * nqq * np + (i * nep + m) * np + k] = ph;
         tab[3 * nqq * np + 3 * nep * np + (i * nep + m) * np + k] = T.edge_w[m] * ph / T.mass[k];
       }
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < np; ++k) {
+      double w = 0.0;
+      for (int m = 0; m < nep; ++m) w += T.edge_w[m] * T.phi_edge[(i * nep + (nep - 1 - m)) * np + k];
+      tab[3 * nqq * np + 6 * nep * np + i * np + k] = w;
+    }
   if (f32) {
     std::vector<float> t32(tab.begin(), tab.end());
     float* p;

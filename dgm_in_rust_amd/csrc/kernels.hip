@@ -61,11 +61,14 @@ __device__ __forceinline__ const double* geo_ed<double>(const DevMesh& m) { retu
 template <>
 __device__ __forceinline__ const float* geo_ed<float>(const DevMesh& m) { return m.ed_geo32; }
 
-// Table offsets (bulk type T): PHIV, DXI, DETA [NQ][NP]; PHIE, LIFT [3][NE][NP].
+// Table offsets (bulk type T): PHIV, DXI, DETA [NQ][NP]; PHIE, LIFT [3][NE][NP]; KXW [3][NP], the edge-weighted
+// basis of face i read in reversed point order, KXW[i][k] = sum_q w_q phi_i(NE - 1 - q, k): the neighbour's
+// side of a KXRCF jump integral as one dot product with its modes.
 template <int NP, int NQ, int NE>
 struct TabOff {
   static constexpr int PHIV = 0, DXI = NQ * NP, DETA = 2 * NQ * NP, PHIE = 3 * NQ * NP, LIFT = 3 * NQ * NP + 3 * NE * NP;
-  static constexpr int N = 3 * NQ * NP + 6 * NE * NP;
+  static constexpr int KXW = 3 * NQ * NP + 6 * NE * NP;
+  static constexpr int N = KXW + 3 * NP;
 };
 // FP64 limiter tables: PHIE [3][NE][NP], EW [NE], CW [NQ], DREF [nkeys][NQ][NP]
 template <int P, int NQ, int NE>
@@ -1092,19 +1095,22 @@ __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U,
     }
     const bool inflow = has && mu * nx + mv * ny < A(0);
     if constexpr (PRELOAD) {
-      A rj = 0, ej = 0;
+      // jump = (len / 2) sum_q w_q (own_q - nb_q): the own side from the traces above, the neighbour's
+      // (its modes on this face's basis in reversed point order) as one dot product with KXW
+      const T* kxw = tphie + (TabOff<NP, NQ, NE>::KXW - TabOff<NP, NQ, NE>::PHIE);
+      A ro = 0, eo = 0, rn = 0, en2 = 0;
 #pragma unroll
       for (int q = 0; q < NE; ++q) {
-        A xr = 0, xe = 0;
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-          const A ph = (A)phie[(lic * NE + (NE - 1 - q)) * NP + k];
-          xr += (A)nrho[lic][k] * ph;
-          xe += (A)nen[lic][k] * ph;
-        }
-        rj += A(0.5) * len * (A)ew[q] * (tr[q] - xr);
-        ej += A(0.5) * len * (A)ew[q] * (te[q] - xe);
+        ro += (A)ew[q] * tr[q];
+        eo += (A)ew[q] * te[q];
       }
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const A w = kxw[lic * NP + k];
+        rn += (A)nrho[lic][k] * w;
+        en2 += (A)nen[lic][k] * w;
+      }
+      const A rj = A(0.5) * len * (ro - rn), ej = A(0.5) * len * (eo - en2);
       djump = inflow ? djump + rj : djump;
       ejump = inflow ? ejump + ej : ejump;
       surface = inflow ? surface + len : surface;
