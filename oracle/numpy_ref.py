@@ -270,6 +270,58 @@ class NumpyRef:
         with np.errstate(invalid="ignore", divide="ignore"):
             return bool(abs(dj) / (h * surf * mr) >= 1.0 or abs(ej) / (h * surf * me) >= 1.0)
 
+    def kxrcf_margin(self, U, e, eps=2.0 ** -24):
+        """kxrcf() above, plus whether the decision is certain under FP32 evaluation of the same state:
+        returns (troubled, certain). An evaluation in single precision (traces as sums of Np products,
+        jumps as sums over the edge points) moves each indicator by at most the bound below; the decision
+        is uncertain when an indicator lies within that bound of 1, or an inflow test within rounding of
+        0. Checker for the FP32-bulk detector; the bound is a generous a-priori one (16 eps per sum)."""
+        ne = len(self.ew)
+        dj = ej = surf = 0.0
+        edj = eej = 0.0
+        mr = me = 1.0e10
+        emr = eme = 0.0
+        certain = True
+        for i in range(3):
+            k = self.el_edges[e, i]
+            v0, v1, L, Rr, il, ir, patch = self.E[k]
+            if Rr == -1:
+                continue
+            nb = self.nbr[e, i]
+            nx, ny = self.n[k] if self.tri[e, i] == v0 else -self.n[k]
+            tr = U[e] @ self.phie[i].T
+            ta = np.abs(U[e]) @ np.abs(self.phie[i]).T            # size of the terms of each trace
+            if tr[0].min() < mr:
+                mr, emr = tr[0].min(), 16 * eps * ta[0][np.argmin(tr[0])]
+            if tr[3].min() < me:
+                me, eme = tr[3].min(), 16 * eps * ta[3][np.argmin(tr[3])]
+            mom, mta = tr[:, ne // 2], ta[:, ne // 2]
+            mn = mom[1] * nx + mom[2] * ny
+            if abs(mn) <= 16 * eps * (mta[1] * abs(nx) + mta[2] * abs(ny)):
+                certain = False
+            if mn < 0:
+                ext = U[nb] @ self.phie[i, ::-1].T
+                xa = np.abs(U[nb]) @ np.abs(self.phie[i, ::-1]).T
+                c = 0.5 * self.len[k] * self.ew
+                dj += float(np.sum(c * (tr[0] - ext[0])))
+                ej += float(np.sum(c * (tr[3] - ext[3])))
+                edj += 16 * eps * float(np.sum(c * (ta[0] + xa[0])))
+                eej += 16 * eps * float(np.sum(c * (ta[3] + xa[3])))
+                surf += self.len[k]
+        h = self.cr[e] ** 1.5
+        if surf == 0.0:
+            return False, certain
+        out = []
+        for jmp, err, mn_, emn in ((dj, edj, mr, emr), (ej, eej, me, eme)):
+            ind = abs(jmp) / (h * surf * mn_)
+            # rounding of the jump, of the minimum and of the products and the division
+            tol = err / (h * surf * abs(mn_)) + ind * (emn / abs(mn_) + 16 * eps)
+            out.append((ind >= 1.0, abs(ind - 1.0) > tol))
+        troubled = out[0][0] or out[1][0]
+        # a certain "troubled" from either indicator settles it; otherwise both must be certain
+        sure = (out[0][0] and out[0][1]) or (out[1][0] and out[1][1]) or (out[0][1] and out[1][1])
+        return troubled, certain and sure
+
     def weno(self, U, e):
         lw = [0.997, 0.001, 0.001, 0.001]
         bnd = [i for i in range(3) if self.nbr[e, i] < 0]

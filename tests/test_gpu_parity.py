@@ -88,6 +88,29 @@ def test_limiter_fp64(all_meshes, name, limiter):
     assert rel_err(V, Uref) <= TOL64, rel_err(V, Uref)
 
 
+@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3", "ogrid_p1"])
+def test_limiter_fp32_bulk_detection(all_meshes, name):
+    """FP32-bulk detection (traces, jumps and indicators in FP32, on the FP32 shadow of the state)
+    against the FP64 decision on that same rounded state: equal for every cell whose decision FP32
+    rounding cannot move (oracle/numpy_ref.py kxrcf_margin), and those cells are nearly all of them."""
+    from oracle.numpy_ref import NumpyRef
+    mesh, order, kw = all_meshes[name]
+    ref, ctx = pair(mesh, order, limiter="jacobi", precision="fp32", **kw)
+    nref = NumpyRef(mesh, ref.tables(), order=order, fs=(kw["mach"], kw["aoa_deg"], 1.0, 1e5, 1.4),
+                    limiter="jacobi")
+    U = flow_state(ref, seed=14, shock=True, mach=kw["mach"], noise=0.02)
+    U32 = U.astype(np.float32).astype(np.float64)
+    t = np.zeros(ctx.ne, np.uint8)
+    SpatialDisc(ctx).apply_limiter(U.copy(), t)
+    dec = [nref.kxrcf_margin(U32, e) for e in range(ctx.ne)]
+    want = np.array([d for d, _ in dec])
+    sure = np.array([c for _, c in dec])
+    got = t.astype(bool)
+    assert want[sure].sum() > 0, "state must trouble some cells"
+    assert np.array_equal(got[sure], want[sure]), np.nonzero(got[sure] != want[sure])[0][:20]
+    assert sure.mean() >= 0.9, sure.mean()
+
+
 @pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3"])
 def test_rk3_step_fp64(all_meshes, name):
     mesh, order, kw = all_meshes[name]

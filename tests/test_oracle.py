@@ -271,6 +271,23 @@ def test_numpy_restatement_kxrcf_quirks():
 
 
 @pytest.mark.parametrize("name", list(SMALL))
+def test_kxrcf_margin_decides_as_kxrcf(name):
+    """The margin-carrying detector (the checker of the FP32-bulk detector) takes the same decisions as
+    kxrcf() and as the C++ oracle, and its rounding band leaves most cells decided with certainty."""
+    fn, order = SMALL[name]
+    ref, nref = both(fn(), order)
+    n = certain = 0
+    for seed in range(3):
+        U = flow_state(ref, seed=seed, shock=True, mach=0.5, noise=0.03)
+        for e in range(ref.ne):
+            t, c = nref.kxrcf_margin(U, e)
+            assert t == nref.kxrcf(U, e) == ref.kxrcf(U, e)
+            n += 1
+            certain += int(c)
+    assert certain >= 0.9 * n, (certain, n)
+
+
+@pytest.mark.parametrize("name", list(SMALL))
 def test_numpy_restatement_time_step(name):
     fn, order = SMALL[name]
     ref, nref = both(fn(), order)
