@@ -17,6 +17,8 @@ import os
 import re
 import sys
 
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak bandwidth, GB/s
+
 
 def kernel_key(name: str) -> str:
     """Kernel name without namespace, argument list and spaces: k_surface<float,10,16,5>."""
@@ -74,15 +76,18 @@ def summarise(out, title, args, copy_mib=256):
              f"# counter scale from the {copy_mib} MiB device copy (reads and writes {kib:.0f} KiB): "
              f"FETCH_SIZE read {fraw:.0f} KiB -> x{ff:.2f}; WRITE_SIZE read {wraw:.0f} KiB -> x{fw:.2f}",
              "# avg/min/max in microseconds per dispatch; fetch/write: raw KiB per dispatch (mean);",
-             "# traffic: (fetch * fetch_scale + write * write_scale) in MB (1e6 bytes) per dispatch",
-             "%-40s %6s %9s %9s %9s %9s %11s %11s %10s" % ("kernel", "calls", "avg_us", "min_us", "max_us",
-                                                          "total_ms", "fetch_kib", "write_kib", "traffic_mb")]
+             "# traffic: (fetch * fetch_scale + write * write_scale) in MB (1e6 bytes) per dispatch;",
+             f"# gb_s: traffic / avg_us; hbm_frac: gb_s / {HBM_PEAK_GBS:.0f} GB/s (MI355X HBM3E peak)",
+             "%-40s %6s %9s %9s %9s %9s %11s %11s %10s %8s %8s" % (
+                 "kernel", "calls", "avg_us", "min_us", "max_us", "total_ms", "fetch_kib", "write_kib", "traffic_mb",
+                 "gb_s", "hbm_frac")]
     for k, r in sorted(stats.items(), key=lambda kv: -kv[1]["tot"]):
         f = sum(fetch[k]) / len(fetch[k]) if fetch.get(k) else float("nan")
         w = sum(write[k]) / len(write[k]) if write.get(k) else float("nan")
         t = (f * ff + w * fw) * 1024.0 / 1e6
-        lines.append("%-40s %6d %9.2f %9.2f %9.2f %9.3f %11.1f %11.1f %10.2f" %
-                     (k, r["calls"], r["avg"], r["mn"], r["mx"], r["tot"], f, w, t))
+        gbs = t * 1e6 / (r["avg"] * 1e-6) / 1e9 if r["avg"] > 0 else float("nan")
+        lines.append("%-40s %6d %9.2f %9.2f %9.2f %9.3f %11.1f %11.1f %10.2f %8.0f %8.3f" %
+                     (k, r["calls"], r["avg"], r["mn"], r["mx"], r["tot"], f, w, t, gbs, gbs / HBM_PEAK_GBS))
     return "\n".join(lines) + "\n"
 
 
@@ -96,7 +101,7 @@ def read_traffic(path, prefix):
         if line.startswith("#"):
             continue
         f = line.split()
-        if len(f) == 9 and f[0].startswith(prefix) and f[8] != "nan":
+        if len(f) in (9, 11) and f[0].startswith(prefix) and f[8] != "nan":
             tr.append(float(f[8]) * 1e6)
             us.append(float(f[2]))
     if not tr:
