@@ -1150,12 +1150,12 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
   if ((e >> 6) >= (m.nes >> 6)) return;
   const int cs = case_of(m, m.el_case64, e);
   CaseCtl* cc = ctl.cases + (cs < 0 ? 0 : cs);
-  // The case record is read here but consulted only after the element's own loads have gone out:
-  // every lane below the capacity loads in bounds (the arrays are capacity-sized, and the connectivity
-  // of unused capacity is -1 or a former case's in-range ids), and the results of a lane outside a
-  // running case's range are masked, not branched around, so the loads stay ahead of the record.
-  // (scalar loads return out of order, so a wait for any one waits for all: the record is read just
-  // after the element's loads are issued, and nothing before them branches on a kernel argument)
+  // The case record is read once the element's own loads have gone out (in after_loads) and consulted
+  // after them: every lane below the capacity loads in bounds (the arrays are capacity-sized, and the
+  // connectivity of unused capacity is -1 or a former case's in-range ids), and the results of a lane
+  // outside a running case's range are masked, not branched around. (Scalar loads return out of order,
+  // so a wait for any one waits for all: nothing before the element's loads branches on a kernel
+  // argument or waits for the record.)
   const double hmin = ld32(m.el_geo + (size_t)G_HMIN * m.nes, (unsigned)e);
   double s0, s1, s2, s3;
   int c_active, e_end, c_step;
@@ -1306,7 +1306,7 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
     const int idx = lane + 64 * it;
     bq[it] = smat[(size_t)e * NPK + (idx < NPK ? idx : 0)];
   }
-  // candidate j, variable v, mode k at idx = (j * 4 + v) * NP + k, three per lane
+  // candidate j, variable v, mode k at idx = (j * 4 + v) * NP + k, NIT per lane
   constexpr int NSRC = 16 * NP, NIT = (NSRC + 63) / 64;
   double sv[NIT];
 #pragma unroll
