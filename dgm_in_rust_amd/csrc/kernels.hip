@@ -133,34 +133,24 @@ __device__ __forceinline__ int hllc(const T lv[4], const T rv[4], T nx, T ny, T 
   const T qlw = (ps <= pl) ? T(1) : sqrt_(T(1) + gk * (dv(ps, pl) - T(1)));
   const T qrw = (ps <= pr) ? T(1) : sqrt_(T(1) + gk * (dv(ps, pr) - T(1)));
   const T sl = ul - cl * qlw, sr = ur + cr * qrw;
-  T f0, fn, ft, f3;
-  if (sl >= T(0)) {
-    f0 = ql1;
-    fn = ql1 * ul + pl;
-    ft = ql2 * ul;
-    f3 = ul * (ql3 + pl);
-  } else if (sr <= T(0)) {
-    f0 = qr1;
-    fn = qr1 * ur + pr;
-    ft = qr2 * ur;
-    f3 = ur * (qr3 + pr);
-  } else {
-    const T ss = dv(pr - pl + ql0 * ul * (sl - ul) - qr0 * ur * (sr - ur), ql0 * (sl - ul) - qr0 * (sr - ur));
-    if (ss >= T(0)) {
-      const T a = dv(ql0 * (sl - ul), sl - ss);
-      const T e3 = a * (ql3 * il + (ss - ul) * (ss + dv(pl, ql0 * (sl - ul))));
-      f0 = ql1 + sl * (a - ql0);
-      fn = (ql1 * ul + pl) + sl * (a * ss - ql1);
-      ft = (ql2 * ul) + sl * (a * vl - ql2);
-      f3 = ul * (ql3 + pl) + sl * (e3 - ql3);
-    } else {
-      const T a = dv(qr0 * (sr - ur), sr - ss);
-      const T e3 = a * (qr3 * ir + (ss - ur) * (ss + dv(pr, qr0 * (sr - ur))));
-      f0 = qr1 + sr * (a - qr0);
-      fn = (qr1 * ur + pr) + sr * (a * ss - qr1);
-      ft = (qr2 * ur) + sr * (a * vr - qr2);
-      f3 = ur * (qr3 + pr) + sr * (e3 - qr3);
-    }
+  // The reference's four cases (:101-147): F(q_L) if S_L >= 0; else F(q_R) if S_R <= 0; else the star
+  // state of side L if S* >= 0, of side R otherwise. Each is F(q_K) of one side K, plus, in the star
+  // region, S_K (U*_K - q_K). The side's values are selected first and the expressions evaluated once,
+  // the same operations as the case taken (lanes of a wavefront take different cases, and evaluating
+  // every case under a mask would cost all four).
+  const T ss = dv(pr - pl + ql0 * ul * (sl - ul) - qr0 * ur * (sr - ur), ql0 * (sl - ul) - qr0 * (sr - ur));
+  const bool star = !(sl >= T(0)) && !(sr <= T(0));
+  const bool left = (sl >= T(0)) || (star && ss >= T(0));
+  const T k0 = left ? ql0 : qr0, k1 = left ? ql1 : qr1, k2 = left ? ql2 : qr2, k3 = left ? ql3 : qr3;
+  const T uk = left ? ul : ur, vk = left ? vl : vr, pk = left ? pl : pr, ik = left ? il : ir, sk = left ? sl : sr;
+  T f0 = k1, fn = k1 * uk + pk, ft = k2 * uk, f3 = uk * (k3 + pk);
+  if (star) {
+    const T a = dv(k0 * (sk - uk), sk - ss);
+    const T e3 = a * (k3 * ik + (ss - uk) * (ss + dv(pk, k0 * (sk - uk))));
+    f0 = k1 + sk * (a - k0);
+    fn = fn + sk * (a * ss - k1);
+    ft = ft + sk * (a * vk - k2);
+    f3 = f3 + sk * (e3 - k3);
   }
   out[0] = f0;
   out[1] = fn * nx - ft * ny;
