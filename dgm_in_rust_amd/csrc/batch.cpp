@@ -41,6 +41,10 @@ struct dgm_batch {
   std::vector<void*> allocs;
   std::unique_ptr<Arena> arena[4];
   std::vector<BatchCase> cases;
+  // one stream per order: with more than one order present, a step runs the arenas' stage sequences
+  // as parallel branches (fork and join through events on the batch stream)
+  hipStream_t astream[4] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[4] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int64_t captured[4][2] = {};            // arena extents the graph was captured with
@@ -108,7 +112,25 @@ void enqueue_step(dgm_batch* b) {
   if (b->prm.limiter != DGM_LIMITER_JACOBI)
     for (int g = 0; g < 4; ++g)
       if (b->arena[g] && b->arena[g]->extent_elements() > 0) b->arena[g]->solver->time_step(b->arena[g]->S, b->stream, 0);
-  // the step control of every case slot runs inside the first arena's stage-0 edge launch
+  int present = 0;
+  for (int g = 0; g < 4; ++g) present += (b->arena[g] && b->arena[g]->extent_elements() > 0) ? 1 : 0;
+  if (present > 1) {
+    // Arenas share nothing but the case records and dt slots, each touching only its own cases': after
+    // the step control (its own launch here, since every arena's stage 0 needs it), each arena's three
+    // stages run on the arena's stream, and the batch stream waits for all of them.
+    launch_step_begin(shared_ctl(b), b->prm.max_cases, b->stream);
+    (void)hipEventRecord(b->fork_ev, b->stream);
+    for (int g = 0; g < 4; ++g) {
+      if (!b->arena[g] || b->arena[g]->extent_elements() <= 0) continue;
+      (void)hipStreamWaitEvent(b->astream[g], b->fork_ev, 0);
+      for (int s = 0; s < 3; ++s) b->arena[g]->solver->stage(b->arena[g]->S, s, b->astream[g], 0);
+      (void)hipEventRecord(b->join_ev[g], b->astream[g]);
+    }
+    for (int g = 0; g < 4; ++g)
+      if (b->arena[g] && b->arena[g]->extent_elements() > 0) (void)hipStreamWaitEvent(b->stream, b->join_ev[g], 0);
+    return;
+  }
+  // one order: the step control of every case slot runs inside its stage-0 edge launch
   int nctl = b->prm.max_cases;
   for (int s = 0; s < 3; ++s)
     for (int g = 0; g < 4; ++g)
@@ -173,6 +195,12 @@ int dgm_batch_create(const dgm_batch_params* params, dgm_batch** out) {
     dgm_batch_destroy(b.release());
     return st;
   };
+  if (hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming) != hipSuccess) return cleanup(DGM_HIP_ERROR);
+  for (int g = 0; g < 4; ++g) {
+    if (P.capacity[g] <= 0) continue;
+    if (hipStreamCreateWithFlags(&b->astream[g], hipStreamNonBlocking) != hipSuccess) return cleanup(DGM_HIP_ERROR);
+    if (hipEventCreateWithFlags(&b->join_ev[g], hipEventDisableTiming) != hipSuccess) return cleanup(DGM_HIP_ERROR);
+  }
   if ((s = balloc(b.get(), &b->d_cases, nc))) return cleanup(s);
   if ((s = balloc(b.get(), &b->d_dtmin, (size_t)nc * DT_CASE))) return cleanup(s);
   if ((s = balloc(b.get(), &b->d_force, 2 * (size_t)nc))) return cleanup(s);
@@ -212,11 +240,18 @@ int dgm_batch_create(const dgm_batch_params* params, dgm_batch** out) {
 void dgm_batch_destroy(dgm_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto st : b->astream)
+    if (st) (void)hipStreamSynchronize(st);
   drop_graph(b);
   for (auto& a : b->arena) a.reset();
   for (void* p : b->allocs) (void)hipFree(p);
   if (b->h_cases) (void)hipHostFree(b->h_cases);
   if (b->h_dt) (void)hipHostFree(b->h_dt);
+  for (int g = 0; g < 4; ++g) {
+    if (b->astream[g]) (void)hipStreamDestroy(b->astream[g]);
+    if (b->join_ev[g]) (void)hipEventDestroy(b->join_ev[g]);
+  }
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
