@@ -169,10 +169,11 @@ def test_final_time_clamp(all_meshes):
 
 
 @pytest.mark.parametrize("variant", ["default", "other"])
-@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3"])
+@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3", "ogrid_p1", "periodic_p4", "box_p2"])
 def test_residual_fp32_bulk(all_meshes, name, variant):
     """Both FP32-bulk element kernels (matrix-core: element_waves > 0; one lane per element: 0) against
-    the oracle, at the order where each is the default and where it is not."""
+    the oracle, at the order where each is the default and where it is not (p = 4 has one lane per
+    element only: its 25 cell points exceed the matrix-core tile), on every mesh family."""
     mesh, order, kw = all_meshes[name]
     ref, ctx = pair(mesh, order, precision="fp32", **kw)
     if variant == "other":
@@ -186,11 +187,14 @@ def test_residual_fp32_bulk(all_meshes, name, variant):
     assert rel_l2(R, Rref) <= 2e-5, rel_l2(R, Rref)
 
 
-def test_time_march_fp32_bulk(all_meshes):
-    mesh, order, kw = all_meshes["ogrid_p3"]
-    ref, ctx = pair(mesh, order, limiter="off", precision="fp32", cfl=0.3, **kw)
+@pytest.mark.parametrize("name,cfl", [("ogrid_p3", 0.3), ("ogrid_p1", 0.0), ("periodic_p4", 0.0)])
+def test_time_march_fp32_bulk(all_meshes, name, cfl):
+    """20 FP32-bulk steps against the FP64 oracle (cfl 0: the reference's CFL formula for the order)."""
+    mesh, order, kw = all_meshes[name]
+    ref, ctx = pair(mesh, order, limiter="off", precision="fp32", cfl=cfl, **kw)
     U0 = flow_state(ref, seed=18, mach=kw["mach"], amp=0.02)
     Uref, s, n, t = ref.time_march(U0, 20, 100.0)
+    assert s == 0 and n == 20
     V = U0.copy()
     TemperalDisc(ctx, final_step=20, final_time=100.0).time_march(SpatialDisc(ctx), ctx.zeros(), V)
     assert rel_l2(V, Uref) <= 1e-6, rel_l2(V, Uref)
