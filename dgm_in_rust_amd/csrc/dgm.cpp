@@ -29,6 +29,7 @@ struct dgm_ctx {
   double* d_force = nullptr;                // [2]
   int* d_flag = nullptr;
   int* d_levels = nullptr;
+  int* d_walk = nullptr;
   std::vector<void*> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -189,6 +190,26 @@ int build_device(dgm_ctx* c) {
   if (!lv.empty()) HIPCHK(c, hipMemcpy(c->d_levels, lv.data(), lv.size() * 4, hipMemcpyHostToDevice));
   A.S.levels = c->d_levels;
   A.S.level_offsets.assign(c->M.level_offsets.begin(), c->M.level_offsets.end());
+  const char* wv = std::getenv("DGM_SEQUENTIAL_WALK");  // "0": one launch per level instead of the walk
+  if (P.limiter == DGM_LIMITER_SEQUENTIAL && !(wv && wv[0] == '0')) {
+    // walk records: {e, lower-numbered neighbours or -1} in level order, each level padded to 64 records
+    std::vector<int> wk;
+    for (size_t l = 0; l + 1 < c->M.level_offsets.size(); ++l) {
+      for (int32_t q = c->M.level_offsets[l]; q < c->M.level_offsets[l + 1]; ++q) {
+        const int e = c->M.level_elems[q];
+        wk.push_back(e);
+        for (int i = 0; i < 3; ++i) {
+          const int64_t n = c->M.el_nbr[3 * (size_t)e + i];
+          wk.push_back(n >= 0 && n < e ? (int)n : -1);
+        }
+      }
+      while (wk.size() % 256) wk.push_back(-1);
+    }
+    if (int s = dalloc(c, &c->d_walk, wk.size())) return s;
+    if (!wk.empty()) HIPCHK(c, hipMemcpy(c->d_walk, wk.data(), wk.size() * 4, hipMemcpyHostToDevice));
+    A.S.walk = c->d_walk;
+    A.S.walk_chunks = (int)(wk.size() / 256);
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DGM_OK;
 }

@@ -1423,10 +1423,12 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
 }
 
 
-// The troubled list holds elements of running cases only (the detector skips the others).
+// The troubled list holds elements of running cases only (the detector skips the others). With defer
+// (the pass ahead of the element-order walk) a failed rebuild is left in pend[e] as -2 - status for
+// the walk to report or drop, instead of stopping the case here.
 template <class T, int P, int NQ, int NE>
 __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U, const double* __restrict__ smat,
-                                             LimiterBufs lb, Ctl ctl, int slot, int fuse_dt) {
+                                             LimiterBufs lb, Ctl ctl, int slot, int fuse_dt, int defer) {
   __shared__ WenoShared<P, NQ, NE> sh;
   constexpr int NP = np_of(P);
   // the block's first list entry is read in the same trip as the count (at a clamped index: the list
@@ -1448,7 +1450,13 @@ __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U,
       step = ctl.cases[cs].step;
     });
     CaseCtl* cc = ctl.cases + cs;
-    if (st && threadIdx.x == 0) atomicMin(&cc->error, (0xFFFFFFull << 8) | (unsigned long long)st);
+    if (st && threadIdx.x == 0) {
+      if (defer) {
+        lb.pend[e] = -2 - st;
+        lb.flags[e] = 2;
+      } else
+        atomicMin(&cc->error, (0xFFFFFFull << 8) | (unsigned long long)st);
+    }
     __syncthreads();
     if (fuse_dt && !st && threadIdx.x == 0) {
       const double d = element_dt<NP>([&](int v, int k) { return sh.fin[v * NP + k]; }, m.hcr, cfl, hmin);
@@ -1485,6 +1493,10 @@ __global__ void k_flush(int* __restrict__ pend, const double* __restrict__ w, do
   }
   pend[e] = -1;
 }
+
+// element-order walk: largest mesh whose bit masks fit in LDS, and record chunks staged per trip
+constexpr int WALK_MAX_E = 1 << 18;
+constexpr int WALK_STAGE = 16;
 
 // exact element-order sweep: one wavefront per element of one level; elements of a level are not
 // adjacent, so each reads final values of lower-numbered neighbours and untouched higher ones.
@@ -1523,6 +1535,170 @@ __global__ __launch_bounds__(64) void k_limit_level(DevMesh m, const int* __rest
     atomicAdd(&ctl.count[slot], 1);
     atomicAdd(&cc->count[slot], 1);
   }
+}
+
+// Element-order sweep in one launch (weno.rs:86-91: cells in index order, each seeing the final values
+// of its lower-numbered neighbours and the untouched ones of its higher-numbered ones). The detector and
+// the rebuild have first run on every cell from the stage state (k_kxrcf, then k_weno with defer: flags
+// 1 = troubled and rebuilt, 2 = troubled and failed). A cell none of whose lower-numbered neighbours was
+// rebuilt sees exactly that state, so its decision and its rebuild are those of the pass: the walk only
+// marks it (k_walk_flush writes the rebuild back afterwards, where pend still holds its slot). A cell
+// with a rebuilt lower-numbered neighbour is detected and rebuilt again from the current state, after
+// writing back the pending rebuilds of those neighbours (a higher-numbered neighbour of a cell is never
+// written before that cell's turn: it comes at a later level than the cell, and is only written back
+// for cells of still later levels).
+// One wavefront walks the level schedule: records {e, lower-numbered neighbours or -1} in level order,
+// each level padded to whole 64-record chunks (e = -1), so a chunk holds cells of one level, none
+// adjacent to another. Which cells are rebuilt, and what the pass found, are bit masks in LDS. Counts
+// end as the number of cells written. A failure stops the case (the error names the lowest-numbered
+// failing cell); the walk still finishes.
+template <class T, int P, int NQ, int NE, int STAGE>
+__global__ __launch_bounds__(64) void k_limit_walk(DevMesh m, const int4* __restrict__ recs, int nchunks,
+                                                   StageBufs<T> b, const T* U, const double* __restrict__ lt,
+                                                   const T* __restrict__ tphie, const double* __restrict__ smat,
+                                                   LimiterBufs lb, Ctl ctl, int slot, int force) {
+  constexpr int NP = np_of(P);
+  __shared__ WenoShared<P, NQ, NE> sh;
+  __shared__ double outv[4 * NP];
+  __shared__ unsigned modb[WALK_MAX_E / 32], jtb[WALK_MAX_E / 32], erb[WALK_MAX_E / 32];
+  __shared__ int4 rs[WALK_STAGE * 64];
+  const int lane = threadIdx.x;
+  CaseCtl* cc = ctl.cases + m.single_case;
+  if (!force && !case_runs(*cc)) return;
+  // the pass's outcome as bit masks: one 32-cell word per lane and trip
+  const int ne = m.ne, nw = (ne + 31) >> 5;
+  for (int w = lane; w < nw; w += 64) {
+    unsigned ok = 0u, bad = 0u;
+    if (32 * w + 32 <= ne) {
+      const unsigned* f4 = reinterpret_cast<const unsigned*>(lb.flags + 32 * w);
+      unsigned v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = f4[i];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const unsigned f = (v[i] >> (8 * j)) & 0xffu;
+          ok |= (f == 1u ? 1u : 0u) << (4 * i + j);
+          bad |= (f == 2u ? 1u : 0u) << (4 * i + j);
+        }
+    } else {
+      for (int e = 32 * w; e < ne; ++e) {
+        const unsigned f = lb.flags[e];
+        ok |= (f == 1u ? 1u : 0u) << (e & 31);
+        bad |= (f == 2u ? 1u : 0u) << (e & 31);
+      }
+    }
+    modb[w] = 0u;
+    jtb[w] = ok;
+    erb[w] = bad;
+  }
+  auto bit = [](const unsigned* mask, int j) { return (mask[j >> 5] >> (j & 31)) & 1u; };
+  for (int c0 = 0; c0 < nchunks; c0 += WALK_STAGE) {
+    const int nc = nchunks - c0 < WALK_STAGE ? nchunks - c0 : WALK_STAGE;
+    __syncthreads();
+    for (int i = lane; i < nc * 64; i += 64) rs[i] = recs[(size_t)c0 * 64 + i];
+    __syncthreads();
+    for (int k = 0; k < nc; ++k) {
+      const int4 r = rs[k * 64 + lane];
+      const int e = r.x;
+      const bool live = e >= 0;
+      const bool ly = r.y >= 0 && bit(modb, r.y), lz = r.z >= 0 && bit(modb, r.z), lw = r.w >= 0 && bit(modb, r.w);
+      const bool dirty = live && (ly || lz || lw);
+      const bool jok = live && bit(jtb, e), jbad = live && bit(erb, e);
+      int delta = 0;  // cells written minus cells the pass counted
+      if (!dirty && jok) atomicOr(&modb[e >> 5], 1u << (e & 31));
+      if (!dirty && jbad) {
+        const int p = lb.pend[e];
+        atomicMin(&cc->error, ((unsigned long long)(unsigned)(e - cc->e_off) << 8) | (unsigned long long)(-2 - p));
+        lb.flags[e] = 1;
+        lb.pend[e] = -1;
+        delta = -1;
+      }
+      const unsigned long long dm = __ballot(dirty);
+      if (dm) {
+        // pending rebuilds of the rebuilt lower-numbered neighbours go in place first
+        const int nb[3] = {ly ? r.y : -1, lz ? r.z : -1, lw ? r.w : -1};
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          const int j = dirty ? nb[f] : -1;
+          const int p = j >= 0 ? lb.pend[j] : -1;
+          if (p >= 0) {
+            const double* w = lb.w + (size_t)p * 4 * NP;
+            double v[4 * NP];
+#pragma unroll
+            for (int c = 0; c < 4 * NP; ++c) v[c] = w[c];
+#pragma unroll
+            for (int c = 0; c < 4 * NP; ++c) store_limited<T, NP, STAGE>(b, m.nes, j, c, v[c]);
+            lb.pend[j] = -1;
+          }
+        }
+        // those and the rebuilds written at earlier levels are read below
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int ef = __shfl(e, __builtin_ctzll(dm));
+        double s0, s1, s2, s3;
+        const bool t = kxrcf<T, P, NQ, NE>(m, U, lt, tphie, dirty ? e : ef, s0, s1, s2, s3, []() {});
+        const bool td = dirty && t;
+        if (dirty) lb.flags[e] = td ? 1 : 0;
+        unsigned long long tm = __ballot(td), ok = 0ull;
+        while (tm) {
+          const int l = __builtin_ctzll(tm);
+          tm &= tm - 1;
+          const int ed = __shfl(e, l);
+          // FP32-bulk stages 0 and 1 store d = w - u^n: u^n is read ahead of the rebuild (4 NP <= 64 lanes)
+          constexpr bool DN = sizeof(T) == 4 && STAGE < 2;
+          static_assert(4 * NP <= 64, "one mode per lane");
+          const size_t ix = (size_t)(lane < 4 * NP ? lane : 0) * m.nes + ed;
+          const double un = DN ? b.un[ix] : 0.0;
+          const int st = weno_wave<T, P, NQ, NE>(m, U, smat, m.hcr, ed, sh, outv, []() {});
+          __syncthreads();
+          if (!st) {
+            if constexpr (DN) {
+              if (lane < 4 * NP) {
+                b.s[ix] = (T)outv[lane];
+                b.d[ix] = (T)(outv[lane] - un);
+              }
+            } else {
+              for (int idx = lane; idx < 4 * NP; idx += 64) store_limited<T, NP, STAGE>(b, m.nes, ed, idx, outv[idx]);
+            }
+            if (lane == 0) atomicOr(&modb[ed >> 5], 1u << (ed & 31));
+            ok |= 1ull << l;
+          } else if (lane == 0) {
+            atomicMin(&cc->error, ((unsigned long long)(unsigned)(ed - cc->e_off) << 8) | (unsigned long long)st);
+          }
+          __syncthreads();
+        }
+        if (dirty) {
+          delta = (int)((ok >> lane) & 1ull) - ((jok || jbad) ? 1 : 0);
+          if (jok || jbad) lb.pend[e] = -1;
+        }
+      }
+      if (delta) {
+        atomicAdd(&ctl.count[slot], delta);
+        atomicAdd(&cc->count[slot], delta);
+      }
+      // the next chunk's mask reads stay after this chunk's updates (one wavefront: LDS is in order)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+}
+
+// writes back the rebuilds the walk left pending (cells whose decision was the detector pass's)
+template <class T, int NP, int STAGE>
+__global__ __launch_bounds__(256) void k_walk_flush(int* __restrict__ pend, const double* __restrict__ w, StageBufs<T> b,
+                                                    int ne, int nes) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int sl = pend[e];
+  if (sl < 0) return;
+  double v[4 * NP];
+#pragma unroll
+  for (int c = 0; c < 4 * NP; ++c) v[c] = w[(size_t)sl * 4 * NP + c];
+#pragma unroll
+  for (int c = 0; c < 4 * NP; ++c) store_limited<T, NP, STAGE>(b, nes, e, c, v[c]);
+  pend[e] = -1;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1741,12 +1917,21 @@ class SolverImpl final : public Solver {
         go(s, k_kxrcf<T, P, NQ, NE>, blocks_for(s.mesh.ne, bs), bs, st, s.mesh, U, s.ltab, (const T*)s.tab + O::PHIE,
            lb, s.ctl, slot, force, fuse, s.tune.remap);
       } else if (part == PART_REBUILD) {
-        go(s, k_weno<T, P, NQ, NE>, s.tune.weno_grid, 64, st, s.mesh, U, s.smat, lb, s.ctl, slot, fuse);
+        go(s, k_weno<T, P, NQ, NE>, s.tune.weno_grid, 64, st, s.mesh, U, s.smat, lb, s.ctl, slot, fuse, 0);
       } else if (part == PART_APPLY && slot == 3) {
         // only the stand-alone limiter writes back at once; in a march the rebuilds stay pending for
         // the next stage's surface and element kernels (and stage 0 writes back u^n)
         go(s, k_apply<T, NP, STAGE>, 1024, 256, st, lb, b, s.mesh.nes, s.ctl, slot);
       }
+    } else if (s.limiter == DGM_LIMITER_SEQUENTIAL && part == PART_DETECT && s.walk && s.lim_flags &&
+               s.mesh.ne <= WALK_MAX_E) {
+      const int bs = s.tune.detect_block;
+      k_kxrcf<T, P, NQ, NE><<<blocks_for(s.mesh.ne, bs), bs, 0, st>>>(s.mesh, U, s.ltab, (const T*)s.tab + O::PHIE, lb,
+                                                                       s.ctl, slot, force, 0, s.tune.remap);
+      k_weno<T, P, NQ, NE><<<s.tune.weno_grid, 64, 0, st>>>(s.mesh, U, s.smat, lb, s.ctl, slot, 0, 1);
+      k_limit_walk<T, P, NQ, NE, STAGE><<<1, 64, 0, st>>>(s.mesh, (const int4*)s.walk, s.walk_chunks, b, U, s.ltab,
+                                                          (const T*)s.tab + O::PHIE, s.smat, lb, s.ctl, slot, force);
+      k_walk_flush<T, NP, STAGE><<<blocks_for(s.mesh.ne, 256), 256, 0, st>>>(s.lim_pend, s.lim_w, b, s.mesh.ne, s.mesh.nes);
     } else if (s.limiter == DGM_LIMITER_SEQUENTIAL && part == PART_DETECT) {
       for (int l = 0; l + 1 < (int)s.level_offsets.size(); ++l) {
         const int n = s.level_offsets[l + 1] - s.level_offsets[l];

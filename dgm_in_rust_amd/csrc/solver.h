@@ -34,6 +34,10 @@ struct DeviceState {
   Tuning tune{};
   std::vector<int> level_offsets;
   const int* levels = nullptr;
+  // element-order walk records {e, lower-numbered neighbours or -1} in level order, each level padded
+  // to 64-record chunks; nullptr: one launch per level instead
+  const int* walk = nullptr;
+  int walk_chunks = 0;
   // when set, the next single-launch stage part records these at its dispatch's start and end
   hipEvent_t prof_ev[2] = {nullptr, nullptr};
 };

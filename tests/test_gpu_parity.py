@@ -88,6 +88,37 @@ def test_limiter_fp64(all_meshes, name, limiter):
     assert rel_err(V, Uref) <= TOL64, rel_err(V, Uref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3", "ogrid_p1", "periodic_p4"])
+def test_sequential_walk_matches_level_launches(all_meshes, name, precision, monkeypatch):
+    """The element-order sweep in one wavefront (after a detector and rebuild pass over every cell)
+    against one launch per level of the schedule: the same troubled cells and the same state, on a
+    state where cells with a rebuilt lower-numbered neighbour are troubled (so the walk re-detects and
+    re-rebuilds them) and where the element order changes the result (it differs from Jacobi's)."""
+    mesh, order, kw = all_meshes[name]
+    ref = RefCase(mesh, order=order, limiter=1, **kw)
+    U = flow_state(ref, seed=14, shock=True, mach=kw["mach"], noise=0.05)
+    out = {}
+    for mode, lim, walk in (("walk", "sequential", "1"), ("levels", "sequential", "0"), ("jacobi", "jacobi", "1")):
+        monkeypatch.setenv("DGM_SEQUENTIAL_WALK", walk)
+        ctx = Context(mesh, order=order, limiter=lim, precision=precision, **kw)
+        V = U.copy()
+        t = np.zeros(ctx.ne, np.uint8)
+        SpatialDisc(ctx).apply_limiter(V, t)
+        out[mode] = (V, t.astype(bool))
+        nbr = ctx.connectivity()["ineighbours"]
+    (Vw, tw), (Vl, tl), (Vj, _) = out["walk"], out["levels"], out["jacobi"]
+    assert np.array_equal(tw, tl)
+    assert rel_err(Vw, Vl) <= (TOL64 if precision == "fp64" else 1e-6), rel_err(Vw, Vl)
+    chained = [e for e in np.nonzero(tw)[0] if any(0 <= j < e and tw[j] for j in nbr[e])]
+    assert len(chained) > 0, "some troubled cell must follow a troubled lower-numbered neighbour"
+    assert rel_err(Vw, Vj) > 1e-9, "the element order must matter on this state"
+    if precision == "fp64":
+        Uref, tref, s = ref.apply_limiter(U)
+        assert s == 0 and np.array_equal(tw, tref)
+        assert rel_err(Vw, Uref) <= TOL64, rel_err(Vw, Uref)
+
+
 @pytest.mark.parametrize("name", ["airfoil_p2", "ogrid_p3", "ogrid_p1"])
 def test_limiter_fp32_bulk_detection(all_meshes, name):
     """FP32-bulk detection (traces, jumps and indicators in FP32, on the FP32 shadow of the state)
