@@ -46,7 +46,10 @@ typedef enum {
 
 enum { DGM_FP64 = 0, DGM_FP32_BULK = 1 };
 /* limiter sweep order: off; the reference's in-place element-order sweep (weno.rs:90,263), run on the
- * device as a level schedule; or Jacobi (every cell tested and rebuilt from the pre-limiter state). */
+ * device as a detector and rebuild pass over every cell, then one wavefront walking the level schedule
+ * that redoes the cells with a rebuilt lower-numbered neighbour (one launch per level when the
+ * environment sets DGM_SEQUENTIAL_WALK=0 at create, or above 2^18 cells); or Jacobi (every cell tested
+ * and rebuilt from the pre-limiter state). */
 enum { DGM_LIMITER_OFF = 0, DGM_LIMITER_SEQUENTIAL = 1, DGM_LIMITER_JACOBI = 2 };
 enum { DGM_WALL = 0, DGM_FARFIELD = 1 };
 
@@ -161,7 +164,8 @@ int dgm_device_clock(dgm_ctx* ctx, double* time, int32_t* steps, double* last_dt
 /* Per-kernel timing: runs n whole steps on the resident state (eagerly, not from the graph) on the
  * context stream and reports the mean milliseconds per launch of each stage part. A part that is one
  * kernel launch is timed by events its dispatch records at its own start and end (hipExtLaunchKernel);
- * a part of several launches (the element-order limiter sweep) by stream events around it. The step
+ * a part of several launches (the element-order limiter: its pass, walk and write-back, all in
+ * detect_ms) by stream events around it. The step
  * control runs inside the stage-0 edge launch, as in a march: surface_ms includes it, and control_ms
  * (kept for the layout) is 0. */
 typedef struct {
