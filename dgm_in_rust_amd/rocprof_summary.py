@@ -1,6 +1,7 @@
 """Text summary of a rocprofv3 kernel-trace run and its FETCH_SIZE / WRITE_SIZE counter runs.
 
 Usage: python -m dgm_in_rust_amd.rocprof_summary OUT "title" "bench arguments" > summary.txt
+       python -m dgm_in_rust_amd.rocprof_summary --trace-only OUT "title" "command" > summary.txt
 
 OUT holds trace/kt_kernel_stats.csv (rocprofv3 --kernel-trace --stats --output-format csv -o kt),
 fetch/pmc_counter_collection.csv and write/pmc_counter_collection.csv (rocprofv3 --pmc FETCH_SIZE /
@@ -91,6 +92,21 @@ def summarise(out, title, args, copy_mib=256):
     return "\n".join(lines) + "\n"
 
 
+def trace_summary(out, title, command):
+    """Per-kernel durations of a kernel-trace run alone (rocprofv3 --kernel-trace --stats --output-format
+    csv -d OUT/trace -o kt -- COMMAND), for runs whose kernels are bound by latency, not by traffic."""
+    stats = read_stats(os.path.join(out, "trace", "kt_kernel_stats.csv"))
+    lines = [f"# {title}",
+             "# command (cd /tmp; export TMPDIR=/tmp; R = repository root):",
+             f"#   rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o kt -- {command}",
+             f"#   python -m dgm_in_rust_amd.rocprof_summary --trace-only OUT",
+             "# avg/min/max in microseconds per dispatch",
+             "%-48s %7s %10s %10s %10s %10s" % ("kernel", "calls", "avg_us", "min_us", "max_us", "total_ms")]
+    for k, r in sorted(stats.items(), key=lambda kv: -kv[1]["tot"]):
+        lines.append("%-48s %7d %10.2f %10.2f %10.2f %10.3f" % (k, r["calls"], r["avg"], r["mn"], r["mx"], r["tot"]))
+    return "\n".join(lines) + "\n"
+
+
 def read_traffic(path, prefix):
     """Mean traffic (bytes per dispatch) and mean duration (us) of the kernels whose key starts with
     prefix, from a summary written by summarise(); None when the file or the kernels are absent."""
@@ -110,6 +126,10 @@ def read_traffic(path, prefix):
 
 
 if __name__ == "__main__":
+    if sys.argv[1] == "--trace-only":
+        sys.stdout.write(trace_summary(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "rocprofv3 summary",
+                                       sys.argv[4] if len(sys.argv) > 4 else "python3 ..."))
+        sys.exit(0)
     out = sys.argv[1]
     title = sys.argv[2] if len(sys.argv) > 2 else "rocprofv3 summary"
     args = sys.argv[3] if len(sys.argv) > 3 else "--steps 200 --warmup 20 --no-cpu --copy-probe 256"

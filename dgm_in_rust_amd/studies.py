@@ -6,12 +6,21 @@ two, the observed convergence order between successive N, the step count and the
 
 python -m dgm_in_rust_amd.studies [--sizes 64,128,256,512] [--orders 1,2,3,4] [--t-end 2.0]
 prints one JSON object per run, then a markdown table.
+
+Limiter-order study (--ogrid): the 160 x 50 NACA 0012 O-grid (16 000 triangles), p = 2, M = 0.5,
+AoA 1.25, CFL 0.3, marched from freestream for --steps steps with the element-order limiter (the
+one-wavefront walk, and one launch per level) and the Jacobi limiter, in FP64 and FP32-bulk. It reports
+ms per step (wall clock of the march, upload and download included), GDOF-stage/s, and how far the
+walk's final state is from the per-level one.
+
+python -m dgm_in_rust_amd.studies --ogrid [--steps 500] [--modes walk,levels,jacobi] [--precisions fp64,fp32]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import math
+import os
 import time
 
 import numpy as np
@@ -33,12 +42,53 @@ def vortex_run(n: int, order: int, precision: str, t_end: float, seed: int = 1):
                         l2_all=float(np.sqrt((err * err).sum())))
 
 
+# limiter-order study modes: (limiter, value of DGM_SEQUENTIAL_WALK read at context creation)
+ORDER_MODES = {"walk": ("sequential", "1"), "levels": ("sequential", "0"), "jacobi": ("jacobi", "1")}
+
+
+def ogrid_order_study(steps: int, modes, precisions):
+    mesh = meshgen.naca0012_ogrid(160, 50, seed=0)
+    rows = []
+    for prec in precisions:
+        final = {}
+        for mode in modes:
+            lim, walk = ORDER_MODES[mode]
+            os.environ["DGM_SEQUENTIAL_WALK"] = walk
+            ctx = Context(mesh, order=2, cfl=0.3, mach=0.5, aoa_deg=1.25, precision=prec, limiter=lim)
+            U0 = ctx.initial_solution(1.0, 1.25, 0.5, 1.0e5)
+            U = U0.copy()  # warm-up: graph capture and first launches
+            TemperalDisc(ctx, final_step=20, final_time=1e9).time_march(SpatialDisc(ctx), ctx.zeros(), U)
+            U = U0.copy()
+            t0 = time.perf_counter()
+            st = TemperalDisc(ctx, final_step=steps, final_time=1e9).time_march(SpatialDisc(ctx), ctx.zeros(), U)
+            wall = time.perf_counter() - t0
+            n = int(st["steps"])
+            row = dict(precision=prec, mode=mode, steps=n, ms_per_step=wall / n * 1e3,
+                       gdof_stage_per_s=ctx.ne * 4 * ctx.nbasis * 3 * n / wall / 1e9)
+            final[mode] = U
+            rows.append(row)
+            print(json.dumps(row), flush=True)
+            ctx.close()
+        os.environ.pop("DGM_SEQUENTIAL_WALK", None)
+        if "walk" in final and "levels" in final:
+            d = float(np.abs(final["walk"] - final["levels"]).max())
+            print(json.dumps(dict(precision=prec, walk_vs_levels_max_abs=d)), flush=True)
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--sizes", default="64,128,256,512")
     ap.add_argument("--orders", default="1,2,3,4")
     ap.add_argument("--t-end", type=float, default=2.0)
+    ap.add_argument("--ogrid", action="store_true", help="the limiter-order study instead of the vortex")
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--modes", default="walk,levels,jacobi")
+    ap.add_argument("--precisions", default="fp64,fp32")
     a = ap.parse_args(argv)
+    if a.ogrid:
+        ogrid_order_study(a.steps, a.modes.split(","), a.precisions.split(","))
+        return
     sizes = [int(x) for x in a.sizes.split(",")]
     orders = [int(x) for x in a.orders.split(",")]
     rows = []
