@@ -100,7 +100,8 @@ struct StageBufs {
 struct LimiterBufs {
   int* list;       // troubled element ids
   double* w;       // reconstructed polynomials [slot][4][Np]
-  uint8_t* flags;  // optional per-element troubled flags
+  uint8_t* flags;  // per-element troubled flags (1 troubled, 0 not); between the element-order limiter's
+                   // detector/rebuild pass and its walk, 2 marks a troubled cell whose rebuild failed
   int* pend;       // per element: its slot in w after the last detector pass, or -1
 };
 
