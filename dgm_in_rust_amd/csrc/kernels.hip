@@ -283,6 +283,29 @@ __device__ __forceinline__ void store_limited(const StageBufs<T>& b, int nes, in
   }
 }
 
+// store a whole rebuilt element w[4 NP]: every load (the row, and u^n where the stage stores d = w - u^n)
+// is issued before the first store, which the compiler could not otherwise move them past
+template <class T, int NP, int STAGE>
+__device__ __forceinline__ void store_limited_row(const StageBufs<T>& b, int nes, int e, const double* w) {
+  constexpr bool DN = sizeof(T) == 4 && STAGE < 2;
+  double v[4 * NP], un[DN ? 4 * NP : 1];
+#pragma unroll
+  for (int c = 0; c < 4 * NP; ++c) {
+    v[c] = w[c];
+    if constexpr (DN) un[c] = b.un[(size_t)c * nes + e];
+  }
+#pragma unroll
+  for (int c = 0; c < 4 * NP; ++c) {
+    const size_t ix = (size_t)c * nes + e;
+    if constexpr (DN) {
+      b.s[ix] = (T)v[c];
+      b.d[ix] = (T)(v[c] - un[c]);
+    } else {
+      store_limited<T, NP, STAGE>(b, nes, e, c, v[c]);
+    }
+  }
+}
+
 // A case that has stopped no longer reads its pending rebuilds, and later limiter passes of other cases
 // reuse their slots: the first stage-0 element pass after it stops writes them back (u^n and the FP32
 // shadow) and clears the marks, before this step's limiter reuses the slots.
@@ -1624,12 +1647,7 @@ __global__ __launch_bounds__(64) void k_limit_walk(DevMesh m, const int4* __rest
           const int j = dirty ? nb[f] : -1;
           const int p = j >= 0 ? lb.pend[j] : -1;
           if (p >= 0) {
-            const double* w = lb.w + (size_t)p * 4 * NP;
-            double v[4 * NP];
-#pragma unroll
-            for (int c = 0; c < 4 * NP; ++c) v[c] = w[c];
-#pragma unroll
-            for (int c = 0; c < 4 * NP; ++c) store_limited<T, NP, STAGE>(b, m.nes, j, c, v[c]);
+            store_limited_row<T, NP, STAGE>(b, m.nes, j, lb.w + (size_t)p * 4 * NP);
             lb.pend[j] = -1;
           }
         }
@@ -1693,11 +1711,7 @@ __global__ __launch_bounds__(256) void k_walk_flush(int* __restrict__ pend, cons
   if (e >= ne) return;
   const int sl = pend[e];
   if (sl < 0) return;
-  double v[4 * NP];
-#pragma unroll
-  for (int c = 0; c < 4 * NP; ++c) v[c] = w[(size_t)sl * 4 * NP + c];
-#pragma unroll
-  for (int c = 0; c < 4 * NP; ++c) store_limited<T, NP, STAGE>(b, nes, e, c, v[c]);
+  store_limited_row<T, NP, STAGE>(b, nes, e, w + (size_t)sl * 4 * NP);
   pend[e] = -1;
 }
 
