@@ -143,3 +143,23 @@ def test_ogrid_c1_100_steps_within_oracle_sensitivity():
     assert 0.0 < spread < 1e-9
     assert rel_l2(U, Uref) <= 1e-12
     assert rel_err(U, Uref) <= 20.0 * spread, (rel_err(U, Uref), spread)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_ogrid_c1_walk_equals_level_launches(precision, monkeypatch):
+    """Config-1 march (160 x 50 O-grid, p = 2, M = 0.5, CFL 0.3), 200 steps from freestream with the
+    element-order limiter: the one-wavefront walk and one launch per level of the schedule give the
+    same state bit for bit, in both precisions, through the impulsive start and past the step where
+    marginal decisions begin to flip (so every re-detection the walk makes is exercised on the real
+    workload, stage after stage)."""
+    mesh = meshgen.naca0012_ogrid(160, 50, seed=0)
+    kw = dict(order=2, cfl=0.3, mach=0.5, aoa_deg=1.25, precision=precision)
+    out = []
+    for walk in ("1", "0"):
+        monkeypatch.setenv("DGM_SEQUENTIAL_WALK", walk)
+        ctx = Context(mesh, limiter="sequential", **kw)
+        U0 = ctx.initial_solution(1.0, 1.25, 0.5, 1.0e5)
+        U, st = march(ctx, U0, 200, 1.0e9)
+        assert st["steps"] == 200
+        out.append(U)
+    assert np.array_equal(out[0], out[1])
