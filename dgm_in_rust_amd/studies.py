@@ -14,6 +14,7 @@ ms per step (wall clock of the march, upload and download included), GDOF-stage/
 walk's final state is from the per-level one.
 
 python -m dgm_in_rust_amd.studies --ogrid [--steps 500] [--modes walk,levels,jacobi] [--precisions fp64,fp32]
+                                  [--grid 160x50] [--order 2] [--mach 0.5]
 """
 from __future__ import annotations
 
@@ -46,16 +47,16 @@ def vortex_run(n: int, order: int, precision: str, t_end: float, seed: int = 1):
 ORDER_MODES = {"walk": ("sequential", "1"), "levels": ("sequential", "0"), "jacobi": ("jacobi", "1")}
 
 
-def ogrid_order_study(steps: int, modes, precisions):
-    mesh = meshgen.naca0012_ogrid(160, 50, seed=0)
+def ogrid_order_study(steps: int, modes, precisions, surface=160, layers=50, order=2, mach=0.5):
+    mesh = meshgen.naca0012_ogrid(surface, layers, seed=0)
     rows = []
     for prec in precisions:
         final = {}
         for mode in modes:
             lim, walk = ORDER_MODES[mode]
             os.environ["DGM_SEQUENTIAL_WALK"] = walk
-            ctx = Context(mesh, order=2, cfl=0.3, mach=0.5, aoa_deg=1.25, precision=prec, limiter=lim)
-            U0 = ctx.initial_solution(1.0, 1.25, 0.5, 1.0e5)
+            ctx = Context(mesh, order=order, cfl=0.3, mach=mach, aoa_deg=1.25, precision=prec, limiter=lim)
+            U0 = ctx.initial_solution(1.0, 1.25, mach, 1.0e5)
             U = U0.copy()  # warm-up: graph capture and first launches
             TemperalDisc(ctx, final_step=20, final_time=1e9).time_march(SpatialDisc(ctx), ctx.zeros(), U)
             U = U0.copy()
@@ -85,9 +86,13 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--modes", default="walk,levels,jacobi")
     ap.add_argument("--precisions", default="fp64,fp32")
+    ap.add_argument("--grid", default="160x50", help="O-grid surface points x layers (--ogrid)")
+    ap.add_argument("--order", type=int, default=2, help="polynomial order (--ogrid)")
+    ap.add_argument("--mach", type=float, default=0.5, help="freestream Mach number (--ogrid)")
     a = ap.parse_args(argv)
     if a.ogrid:
-        ogrid_order_study(a.steps, a.modes.split(","), a.precisions.split(","))
+        ns, nl = (int(x) for x in a.grid.split("x"))
+        ogrid_order_study(a.steps, a.modes.split(","), a.precisions.split(","), ns, nl, a.order, a.mach)
         return
     sizes = [int(x) for x in a.sizes.split(",")]
     orders = [int(x) for x in a.orders.split(",")]
