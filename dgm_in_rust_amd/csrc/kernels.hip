@@ -888,7 +888,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 }
 
 // ---------------------------------------------------------------------------------------
-// time step: TemperalDisc::compute_time_step (wave speed from the sum of ALL modes, Q5)
+// time step: TemperalDisc::compute_time_step (wave speed from the sum of ALL modes, src/temporal_disc.rs:63-79)
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll

@@ -67,8 +67,9 @@ def test_hllc_status_codes():
 
 
 def test_hllc_orientation_dependence_exists():
-    """Quirk Q18: with the altered p* the flux is not antisymmetric under (L,R,n) -> (R,L,-n) for some
-    strongly compressive pairs; the GPU therefore evaluates each edge once, in the reference's orientation."""
+    """With the reference's altered p* (src/spatial_disc/flux.rs:75-78) the flux is not antisymmetric
+    under (L,R,n) -> (R,L,-n) for some strongly compressive pairs; the GPU therefore evaluates each edge
+    once, in the reference's orientation."""
     rng = np.random.default_rng(2)
     asym = 0
     for _ in range(2000):
@@ -263,7 +264,7 @@ def test_numpy_restatement_limiter(name, limiter):
 
 
 def test_numpy_restatement_kxrcf_quirks():
-    """Cells without an inflow face give 0/0 = NaN -> not troubled (Q8)."""
+    """Cells without an inflow face give 0/0 = NaN -> not troubled (src/spatial_disc/limiter/weno.rs:7-85)."""
     ref, nref = both(meshgen.box_farfield(3, seed=5), 2)
     U = ref.initial_solution(1.0, 0.0, 0.0, 1e5)   # fluid at rest: no inflow anywhere
     for e in range(ref.ne):

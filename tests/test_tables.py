@@ -37,7 +37,7 @@ def test_triangle_rule_exactness(tiny_mesh, p, nq, ne):
     deg = DEGREE[nq]
     # the reference's 6-point literals carry 14 digits (weights sum to 0.99999999999999)
     tol = 2e-14 if nq == 6 else 1e-15
-    scale = 0.5 if nq == 4 else 1.0  # src/gauss_point.rs:68: the 4-point weights sum to 0.5 (quirk Q2)
+    scale = 0.5 if nq == 4 else 1.0  # src/gauss_point.rs:68: the 4-point weights sum to 0.5 (a quirk of the reference's rule)
     for a in range(deg + 1):
         for b in range(deg + 1 - a):
             got = float(np.sum(w * pts[:, 0] ** a * pts[:, 1] ** b))
