@@ -89,7 +89,7 @@ def test_lift_matches_host_integral():
 
 
 def test_ogrid_c1_short_run_matches_oracle():
-    """Config-1 mesh (160 x 50 O-grid, 16k triangles), p = 2, M = 0.5, AoA 1.25, CFL 0.3, sequential
+    """The 160 x 50 O-grid (16k triangles), p = 2, M = 0.5, AoA 1.25, CFL 0.3, sequential
     WENO, marched from freestream against the oracle.
 
     The impulsive start makes the wall-layer cells troubled, and WENO's nonlinear weights there
@@ -122,7 +122,7 @@ def test_ogrid_c1_short_run_matches_oracle():
 
 
 def test_ogrid_c1_100_steps_within_oracle_sensitivity():
-    """Config-1 march to step 100 (before any limiter decision near the KXRCF threshold flips). The
+    """The 16k-triangle O-grid march (p = 2, M = 0.5, CFL 0.3) to step 100 (before any limiter decision near the KXRCF threshold flips). The
     state is held to 1e-12 in relative L2, and in the per-variable max norm to 20 times the oracle's
     own spread under one-ulp perturbations of its initial state. (Between steps 100 and 200 a
     marginal troubled-cell decision flips and the trajectory becomes chaotic: the oracle's own one-ulp
@@ -147,7 +147,7 @@ def test_ogrid_c1_100_steps_within_oracle_sensitivity():
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_ogrid_c1_walk_equals_level_launches(precision, monkeypatch):
-    """Config-1 march (160 x 50 O-grid, p = 2, M = 0.5, CFL 0.3), 200 steps from freestream with the
+    """The 160 x 50 O-grid march (16k triangles, p = 2, M = 0.5, CFL 0.3), 200 steps from freestream with the
     element-order limiter: the one-wavefront walk and one launch per level of the schedule give the
     same state bit for bit, in both precisions, through the impulsive start and past the step where
     marginal decisions begin to flip (so every re-detection the walk makes is exercised on the real
