@@ -88,7 +88,7 @@ def test_lift_matches_host_integral():
     assert abs(cl - expect) <= 1e-10 * max(1.0, abs(expect))
 
 
-def test_ogrid_c1_short_run_matches_oracle():
+def test_ogrid_16k_short_run_matches_oracle():
     """The 160 x 50 O-grid (16k triangles), p = 2, M = 0.5, AoA 1.25, CFL 0.3, sequential
     WENO, marched from freestream against the oracle.
 
@@ -121,9 +121,9 @@ def test_ogrid_c1_short_run_matches_oracle():
     assert rel_err(U, Uref) <= 10.0 * spread, (rel_err(U, Uref), spread)
 
 
-def test_ogrid_c1_100_steps_within_oracle_sensitivity():
-    """The 16k-triangle O-grid march (p = 2, M = 0.5, CFL 0.3) to step 100 (before any limiter decision near the KXRCF threshold flips). The
-    state is held to 1e-12 in relative L2, and in the per-variable max norm to 20 times the oracle's
+def test_ogrid_16k_100_steps_within_oracle_sensitivity():
+    """The 16k-triangle O-grid march (p = 2, M = 0.5, CFL 0.3) to step 100, before any limiter decision
+    near the KXRCF threshold flips. The state is held to 1e-12 in relative L2, and in the per-variable max norm to 20 times the oracle's
     own spread under one-ulp perturbations of its initial state. (Between steps 100 and 200 a
     marginal troubled-cell decision flips and the trajectory becomes chaotic: the oracle's own one-ulp
     spread grows to about 1e-1 by step 500.)"""
@@ -146,7 +146,7 @@ def test_ogrid_c1_100_steps_within_oracle_sensitivity():
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_ogrid_c1_walk_equals_level_launches(precision, monkeypatch):
+def test_ogrid_16k_walk_equals_level_launches(precision, monkeypatch):
     """The 160 x 50 O-grid march (16k triangles, p = 2, M = 0.5, CFL 0.3), 200 steps from freestream with the
     element-order limiter: the one-wavefront walk and one launch per level of the schedule give the
     same state bit for bit, in both precisions, through the impulsive start and past the step where
