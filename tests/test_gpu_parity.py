@@ -275,3 +275,32 @@ def test_march_stage0_failure_status(all_meshes, precision):
         TemperalDisc(ctx, final_step=5, final_time=1.0e300).time_march(SpatialDisc(ctx), ctx.zeros(), V)
     assert ei.value.status == 2 and ei.value.edge == bad
     assert np.array_equal(V, U)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_troubled_counts_and_profile_by_limiter_order(all_meshes, precision, monkeypatch):
+    """The troubled-cell counts the library reports. The stand-alone limiter's count equals its flags.
+    After two marched steps the per-stage counts agree between the one-wavefront walk and one launch
+    per level (the walk corrects the detector pass's count to the cells it wrote). Per-kernel timing
+    reports the edge, element and detector parts of a step in both forms."""
+    mesh, order, kw = all_meshes["ogrid_p3"]
+    ref = RefCase(mesh, order=order, limiter=1, **kw)
+    U = flow_state(ref, seed=14, shock=True, mach=kw["mach"], noise=0.05)
+    Um = flow_state(ref, seed=15, shock=True, mach=kw["mach"], amp=0.02)  # marchable: 24 troubled cells
+    counts, states = {}, {}
+    for walk in ("1", "0"):
+        monkeypatch.setenv("DGM_SEQUENTIAL_WALK", walk)
+        ctx = Context(mesh, order=order, limiter="sequential", precision=precision, **kw)
+        t = np.zeros(ctx.ne, np.uint8)
+        SpatialDisc(ctx).apply_limiter(U.copy(), t)
+        c = ctx.troubled_counts()
+        assert c[3] == int(t.astype(bool).sum()) > 0
+        ctx.upload(Um)
+        st = ctx.device_time_march(2, 1.0e9)
+        assert st["steps"] == 2
+        counts[walk] = ctx.troubled_counts()[:3]
+        states[walk] = ctx.download()
+        prof = ctx.profile_steps(2)
+        assert prof["surface_ms"] > 0 and prof["element_ms"] > 0 and prof["detect_ms"] > 0, prof
+    assert counts["1"] == counts["0"] and sum(counts["1"]) > 0, counts
+    assert np.array_equal(states["1"], states["0"])
