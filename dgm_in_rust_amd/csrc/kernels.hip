@@ -1519,7 +1519,7 @@ __global__ void k_flush(int* __restrict__ pend, const double* __restrict__ w, do
 
 // element-order walk: largest mesh whose bit masks fit in LDS, and record chunks staged per trip
 constexpr int WALK_MAX_E = 1 << 18;
-constexpr int WALK_STAGE = 16;
+constexpr int WALK_STAGE = 8;
 
 // exact element-order sweep: one wavefront per element of one level; elements of a level are not
 // adjacent, so each reads final values of lower-numbered neighbours and untouched higher ones.
@@ -1617,11 +1617,25 @@ __global__ __launch_bounds__(64) void k_limit_walk(DevMesh m, const int4* __rest
     erb[w] = bad;
   }
   auto bit = [](const unsigned* mask, int j) { return (mask[j >> 5] >> (j & 31)) & 1u; };
+  if (nchunks <= 0) return;
+  // the next WALK_STAGE chunks' records are loaded into registers while the current ones are walked
+  // (a chunk past the end reads the last one again, unused)
+  typedef int v4i __attribute__((ext_vector_type(4)));  // a native vector stays in registers
+  const v4i* rv = reinterpret_cast<const v4i*>(recs);
+  v4i pf[WALK_STAGE];
+#pragma unroll
+  for (int i = 0; i < WALK_STAGE; ++i) pf[i] = rv[(size_t)(i < nchunks ? i : nchunks - 1) * 64 + lane];
   for (int c0 = 0; c0 < nchunks; c0 += WALK_STAGE) {
     const int nc = nchunks - c0 < WALK_STAGE ? nchunks - c0 : WALK_STAGE;
     __syncthreads();
-    for (int i = lane; i < nc * 64; i += 64) rs[i] = recs[(size_t)c0 * 64 + i];
+#pragma unroll
+    for (int i = 0; i < WALK_STAGE; ++i) rs[i * 64 + lane] = make_int4(pf[i].x, pf[i].y, pf[i].z, pf[i].w);
     __syncthreads();
+#pragma unroll
+    for (int i = 0; i < WALK_STAGE; ++i) {
+      const int ch = c0 + WALK_STAGE + i < nchunks ? c0 + WALK_STAGE + i : nchunks - 1;
+      pf[i] = rv[(size_t)ch * 64 + lane];
+    }
     for (int k = 0; k < nc; ++k) {
       const int4 r = rs[k * 64 + lane];
       const int e = r.x;
