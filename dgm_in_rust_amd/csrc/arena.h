@@ -121,7 +121,7 @@ Tuning to_tuning(const dgm_tuning& t);
 dgm_tuning from_tuning(const Tuning& u);
 bool valid_tuning(const dgm_tuning& t);
 
-// CFL of the reference for order p (src/initialization.rs:128-132), or the given value when > 0.
+// CFL of the reference for order p (src/initialization.rs:29-33), or the given value when > 0.
 double default_cfl(int order, double cfl);
 
 }  // namespace dgm

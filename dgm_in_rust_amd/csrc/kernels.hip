@@ -1007,6 +1007,9 @@ template <class T, int P, int NQ, int NE, class AfterLoads>
 __device__ __forceinline__ bool kxrcf(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ lt,
                                       const T* __restrict__ tphie, int e, double& sum0, double& sum1, double& sum2,
                                       double& sum3, AfterLoads after_loads) {
+  // multiply-adds are fused within one expression only, so every kernel that inlines the detector (the
+  // detector pass, the per-level sweep, the walk) evaluates the same operations in the same order
+#pragma clang fp contract(on)
   using LO = LimOff<P, NQ, NE>;
   constexpr int NP = LO::NP;
   const int nes = m.nes;
@@ -1301,6 +1304,8 @@ struct WenoShared {
 template <class T, int P, int NQ, int NE, class AfterLoads>
 __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double* __restrict__ smat, double hcr, int e,
                          WenoShared<P, NQ, NE>& sh, double* __restrict__ out /* [4][NP] */, AfterLoads after_loads) {
+  // fused within one expression only (see kxrcf): the walk's rebuilds equal the pass's and the levels'
+#pragma clang fp contract(on)
   constexpr int NP = np_of(P);
   constexpr int NPK = NP * (NP + 1) / 2;
   const int lane = threadIdx.x;
@@ -1448,7 +1453,12 @@ __device__ int weno_wave(const DevMesh& m, const T* __restrict__ U, const double
 
 // The troubled list holds elements of running cases only (the detector skips the others). With defer
 // (the pass ahead of the element-order walk) a failed rebuild is left in pend[e] as -2 - status for
-// the walk to report or drop, instead of stopping the case here.
+// the walk to report or drop, instead of stopping the case here. The only failure of weno_wave is a
+// cell with three boundary faces (status 5), and the detector never marks such a cell: with no
+// neighbour face the inflow surface and both jumps are 0 and the indicators 0/0 = NaN, which is not
+// >= 1. So neither this failure path nor the walk's handling of it (flag 2, erb, the -2 - p decoding)
+// can run from a detector pass; they are kept so that a rebuild failure, should one become possible,
+// still stops the case with its status instead of writing an unfinished polynomial.
 template <class T, int P, int NQ, int NE>
 __global__ __launch_bounds__(64) void k_weno(DevMesh m, const T* __restrict__ U, const double* __restrict__ smat,
                                              LimiterBufs lb, Ctl ctl, int slot, int fuse_dt, int defer) {

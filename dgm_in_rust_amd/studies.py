@@ -50,6 +50,19 @@ ORDER_MODES = {"walk": ("sequential", "1"), "levels": ("sequential", "0"), "jaco
 def ogrid_order_study(steps: int, modes, precisions, surface=160, layers=50, order=2, mach=0.5):
     mesh = meshgen.naca0012_ogrid(surface, layers, seed=0)
     rows = []
+    saved = os.environ.get("DGM_SEQUENTIAL_WALK")
+    try:
+        _order_rows(mesh, steps, modes, precisions, order, mach, rows)
+    finally:
+        # the walk choice is read at context creation: later contexts of this process get the caller's
+        if saved is None:
+            os.environ.pop("DGM_SEQUENTIAL_WALK", None)
+        else:
+            os.environ["DGM_SEQUENTIAL_WALK"] = saved
+    return rows
+
+
+def _order_rows(mesh, steps, modes, precisions, order, mach, rows):
     for prec in precisions:
         final = {}
         for mode in modes:

@@ -15,7 +15,7 @@
  *   Solver::set_initial_solution    src/solver.rs:45-79                 -> dgm_initial_solution
  *
  * Host arrays use the reference layout: solutions / residuals are C-order [n_elements][4][n_basis]
- * f64 (ndarray Array<f64, Ix3>, src/initialization.rs:169-170). All calls return a dgm_status;
+ * f64 (ndarray Array<f64, Ix3>, src/initialization.rs:70-71). All calls return a dgm_status;
  * none aborts. Where the reference panics (negative density or pressure in HLLC,
  * src/spatial_disc.rs:189-201; unsupported rule sizes, src/gauss_point.rs:56-58,90-92; more than
  * two boundary faces in WENO, src/spatial_disc/limiter/weno.rs:107), the call returns the
@@ -75,7 +75,7 @@ typedef struct {
   int32_t n_cell_gp;  /* 0 = default: 6 for p <= 2, 16 for p = 3, 25 for p = 4 */
   int32_t n_edge_gp;  /* 0 = default: 3 for p <= 2, 5 for p >= 3 */
   double hcr;         /* gamma */
-  double cfl;         /* <= 0: 1/((2p+1)(1+4/(p+2)^2)), src/initialization.rs:128-132 */
+  double cfl;         /* <= 0: 1/((2p+1)(1+4/(p+2)^2)), src/initialization.rs:29-33 */
   double fs_mach, fs_aoa_deg, fs_rho, fs_p, fs_hcr; /* far-field state, src/initialization.rs:124-133 */
   int32_t precision;  /* DGM_FP64 or DGM_FP32_BULK */
   int32_t limiter;    /* DGM_LIMITER_* */

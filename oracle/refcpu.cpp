@@ -34,6 +34,27 @@ namespace {
 
 enum Status { OK = 0, NEG_DENSITY = 1, NEG_PRESSURE = 2, NONFINITE = 3, BAD_CONFIG = 4, BAD_MESH = 5 };
 
+// Planted mistakes, for the tests that pin the quirk-bearing steps: a build with -DORACLE_PLANT=n
+// (oracle/Makefile target `plants`, libraries librefcpu_plant<n>.so) replaces one quirk with the
+// plausible alternative named below, and each closed-form test must fail on the matching build. The
+// oracle itself is built with ORACLE_PLANT = 0, where every hook below compiles away.
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+enum Plant {
+  PLANT_NONE = 0,
+  PLANT_DT_MODE0 = 1,             // time-step wave speed from mode 0 (the mean) instead of the sum of modes
+  PLANT_HLLC_TEXTBOOK_PSTAR = 2,  // p* numerator c_L + c_R - ((gamma - 1)/2)(u_R - u_L)
+  PLANT_KXRCF_NB_OWN_EDGE = 3,    // neighbour trace on the neighbour's own local edge
+  PLANT_KXRCF_H1 = 4,             // indicator scaled by h instead of h^1.5
+  PLANT_KXRCF_OUTFLOW = 5,        // jumps over outflow faces instead of inflow ones
+  PLANT_KXRCF_ALL_SURFACE = 6,    // surface summed over every face with a neighbour
+  PLANT_WENO_WEIGHT_NEXT = 7,     // a boundary face i zeroes weights[i + 1] (its neighbour's candidate)
+  PLANT_WENO_NO_MEAN_SWAP = 8,    // neighbour candidates keep their own mode 0
+  PLANT_WENO_PLAIN_MEAN = 9,      // unweighted mean of the per-face reconstructions
+};
+constexpr int kPlant = ORACLE_PLANT;
+
 // ---------------------------------------------------------------------------------------
 // Quadrature — src/gauss_point.rs
 // ---------------------------------------------------------------------------------------
@@ -683,9 +704,9 @@ int hllc(const double* lv, const double* rv, const double* normal, double hcr, d
   fr[2] = qr[2] * ur;
   fr[3] = ur * (qr[3] + pr);
   const double zeta = (hcr - 1.0) / (2.0 * hcr);
+  const double pcoef = kPlant == PLANT_HLLC_TEXTBOOK_PSTAR ? (hcr - 1.0) / 2.0 : (zeta - 1.0) / 2.0;
   const double p_star =
-      std::pow((cl + cr - (zeta - 1.0) / 2.0 * (ur - ul)) / (cl / std::pow(pl, zeta) + cr / std::pow(pr, zeta)),
-               1.0 / zeta);
+      std::pow((cl + cr - pcoef * (ur - ul)) / (cl / std::pow(pl, zeta) + cr / std::pow(pr, zeta)), 1.0 / zeta);
   double ql_lower, qr_lower;
   if (p_star <= pl)
     ql_lower = 1.0;
@@ -1029,11 +1050,11 @@ bool modified_kxrcf(const Case& C, const double* U, int64_t ielem) {
     if (el.ivertices[lic] == ed.ivertices[0]) {
       nx = ed.normal[0];
       ny = ed.normal[1];
-      nlic = ed.in_cell_indices[0];
+      nlic = ed.in_cell_indices[kPlant == PLANT_KXRCF_NB_OWN_EDGE ? 1 : 0];
     } else {
       nx = -ed.normal[0];
       ny = -ed.normal[1];
-      nlic = ed.in_cell_indices[1];
+      nlic = ed.in_cell_indices[kPlant == PLANT_KXRCF_NB_OWN_EDGE ? 0 : 1];
     }
     for (int igp = 0; igp < ngp; ++igp) {
       double rho = 0.0, rho_e = 0.0;
@@ -1051,7 +1072,8 @@ bool modified_kxrcf(const Case& C, const double* U, int64_t ielem) {
       rho_v += U[C.idx(ielem, 2, k)] * T.phi_edge[(lic * ngp + cg) * np + k];
     }
     const double center = rho_u * nx + rho_v * ny;
-    if (center < 0.0) {
+    if (kPlant == PLANT_KXRCF_ALL_SURFACE && !(center < 0.0)) surface += ed.jacob_det;
+    if (kPlant == PLANT_KXRCF_OUTFLOW ? center > 0.0 : center < 0.0) {
       double rj = 0.0, ej = 0.0;
       for (int igp = 0; igp < ngp; ++igp) {
         double ir = 0.0, ie = 0.0, xr = 0.0, xe = 0.0;
@@ -1070,8 +1092,9 @@ bool modified_kxrcf(const Case& C, const double* U, int64_t ielem) {
     }
   }
   const double h = el.circumradius;
-  const double ind_d = std::fabs(density_jump) / (std::pow(h, 1.5) * surface * min_density);
-  const double ind_e = std::fabs(total_energy_jump) / (std::pow(h, 1.5) * surface * min_total_energy);
+  const double hp = kPlant == PLANT_KXRCF_H1 ? h : std::pow(h, 1.5);
+  const double ind_d = std::fabs(density_jump) / (hp * surface * min_density);
+  const double ind_e = std::fabs(total_energy_jump) / (hp * surface * min_total_energy);
   return (ind_d >= 1.0) || (ind_e >= 1.0);
 }
 
@@ -1089,7 +1112,7 @@ int weno_element(const Case& C, const double* U, int64_t ielem, double* out) {
   for (int i = 0; i < 3; ++i)
     if (C.edges[el.iedges[i]].ielements[1] == -1) {
       nbnd += 1;
-      lw[i] = 0.0;
+      lw[kPlant == PLANT_WENO_WEIGHT_NEXT ? i + 1 : i] = 0.0;
     }
   if (nbnd == 0)
     lw[0] = 0.997;
@@ -1124,7 +1147,7 @@ int weno_element(const Case& C, const double* U, int64_t ielem, double* out) {
     const Edge& ei = C.edges[el.iedges[i]];
     if (ei.ielements[1] == -1) continue;
     const int64_t inb = el.ineighbours[i];
-    neighbour_area += C.elements[inb].jacob_det * 0.5;
+    neighbour_area += kPlant == PLANT_WENO_PLAIN_MEAN ? 1.0 : C.elements[inb].jacob_det * 0.5;
     double nlw[4][4] = {{0}};  // [poly j][var]
     std::fill(proj.begin(), proj.end(), 0.0);
     const double nx = ei.normal[0], ny = ei.normal[1];  // raw edge normal (:121-122)
@@ -1142,7 +1165,8 @@ int weno_element(const Case& C, const double* U, int64_t ielem, double* out) {
       const int64_t jnb = el.ineighbours[j];
       for (int v = 0; v < 4; ++v)
         for (int k = 0; k < np; ++k) nbp[v * np + k] = U[C.idx(jnb, v, k)];
-      for (int v = 0; v < 4; ++v) nbp[v * np + 0] = tgt[v * np + 0];  // :148-154
+      if (kPlant != PLANT_WENO_NO_MEAN_SWAP)
+        for (int v = 0; v < 4; ++v) nbp[v * np + 0] = tgt[v * np + 0];  // :148-154
       gemm4(Lm, nbp.data(), np, &proj[(j + 1) * 4 * np], use_fma);
       for (int v = 0; v < 4; ++v) {
         const double beta = beta_of(&proj[((j + 1) * 4 + v) * np]);
@@ -1160,7 +1184,7 @@ int weno_element(const Case& C, const double* U, int64_t ielem, double* out) {
       for (int j = 0; j < 4; ++j)
         for (int k = 0; k < np; ++k) pn[v * np + k] += wts[j][v] * proj[(j * 4 + v) * np + k];
     gemm4(Rm, pn.data(), np, pn2.data(), use_fma);
-    const double Jn = C.elements[inb].jacob_det;
+    const double Jn = kPlant == PLANT_WENO_PLAIN_MEAN ? 2.0 : C.elements[inb].jacob_det;
     for (int t = 0; t < 4 * np; ++t) rec[t] = rec[t] + pn2[t] * Jn * 0.5;
   }
   for (int t = 0; t < 4 * np; ++t) out[t] = rec[t] / neighbour_area;
@@ -1220,7 +1244,7 @@ double compute_time_step(const Case& C, const double* U) {
   double time_step = 1.0e10;
   for (size_t e = 0; e < C.elements.size(); ++e) {
     double rho = 0.0, rho_u = 0.0, rho_v = 0.0, rho_e = 0.0;
-    for (int k = 0; k < np; ++k) {
+    for (int k = 0; k < (kPlant == PLANT_DT_MODE0 ? 1 : np); ++k) {
       rho += U[C.idx(e, 0, k)];
       rho_u += U[C.idx(e, 1, k)];
       rho_v += U[C.idx(e, 2, k)];
@@ -1432,7 +1456,7 @@ ref_case* ref_create(int64_t nv, const double* vxy, int64_t neb, const int32_t* 
     return nullptr;
   }
   compute_geometry(C);
-  // CFL, src/initialization.rs:128-132 (the JSON cfl is ignored by the reference; a positive
+  // CFL, src/initialization.rs:29-33 (the JSON cfl is ignored by the reference; a positive
   // ref_params.cfl overrides the formula, as the benchmark configurations require)
   if (prm->cfl > 0.0) {
     C.cfl = prm->cfl;
@@ -1514,6 +1538,7 @@ void ref_get_tables(const ref_case* rc, double* phi_cell, double* phi_edge, doub
 }
 
 double ref_cfl(const ref_case* rc) { return rc->c.cfl; }
+int64_t ref_last_bad_index(const ref_case* rc) { return rc->c.bad_index; }
 
 double ref_phys_deriv(const ref_case* rc, int64_t e, int32_t igp, int32_t k, int32_t l1, int32_t l2) {
   return rc->c.dphys(e, igp, k, l1, l2);

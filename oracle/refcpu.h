@@ -38,7 +38,7 @@ typedef struct {
   int32_t n_cell_gp;    /* triangle rule: 4, 6, 16, 25 */
   int32_t n_edge_gp;    /* Gauss-Legendre rule: 3, 4, 5 */
   double hcr;           /* ratio of specific heats */
-  double cfl;           /* <= 0: the reference formula of src/initialization.rs:128-132 */
+  double cfl;           /* <= 0: the reference formula of src/initialization.rs:29-33 */
   double fs_mach, fs_aoa_deg, fs_rho, fs_p, fs_hcr; /* freestream.json fields (far-field patch state) */
   int32_t limiter;      /* 0 off, 1 sequential in-place (reference order), 2 Jacobi (all cells from the pre-limiter state) */
   int32_t nthreads;     /* residual folds: 1 = deterministic sequential order */
@@ -67,6 +67,8 @@ int32_t ref_n_deriv_keys(const ref_case* c);
 void ref_get_tables(const ref_case* c, double* phi_cell, double* phi_edge, double* mass, double* cell_pts,
                     double* cell_w, double* edge_pts, double* edge_w, double* dref);
 double ref_cfl(const ref_case* c);
+/* edge (or element) at which the last failing residual fold stopped: the first failing entry of its list */
+int64_t ref_last_bad_index(const ref_case* c);
 /* physical derivative d^(l1+l2) phi_k / dx^l1 dy^l2 at cell point igp of element e (src/mesh.rs:85-132) */
 double ref_phys_deriv(const ref_case* c, int64_t e, int32_t igp, int32_t k, int32_t l1, int32_t l2);
 

@@ -27,22 +27,31 @@ class RefParams(C.Structure):
                 ("gemm_fma", C.c_int32)]
 
 
-def build(force: bool = False) -> str:
-    src = os.path.join(HERE, "refcpu.cpp")
-    if force or not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
-        subprocess.run(["make", "-s", "-C", HERE], check=True)
+def plant_path(plant: int) -> str:
+    return os.path.join(HERE, "_build", f"librefcpu_plant{int(plant)}.so")
+
+
+def build(force: bool = False, plants: bool = False) -> str:
+    """Builds the oracle (and, with plants, its planted-mistake variants) when a source is newer."""
+    src = [os.path.join(HERE, f) for f in ("refcpu.cpp", "refcpu.h")]
+    newest = max(os.path.getmtime(f) for f in src)
+    targets = [LIB] + ([plant_path(n) for n in range(1, 10)] if plants else [])
+    stale = [t for t in targets if not os.path.exists(t) or os.path.getmtime(t) < newest]
+    if force or stale:
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "all"] + (["plants"] if plants else []), check=True)
     return LIB
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            build()
-        L = C.CDLL(LIB)
+def lib(plant: int = 0):
+    """The oracle library, or with plant = n its planted-mistake variant n (tests only)."""
+    if plant not in _libs:
+        path = LIB if plant == 0 else plant_path(plant)
+        if not os.path.exists(path):
+            build(plants=plant != 0)
+        L = C.CDLL(path)
         L.ref_gmsh_parse.restype = C.c_void_p
         L.ref_gmsh_parse.argtypes = [C.c_char_p]
         L.ref_gmsh_sizes.argtypes = [C.c_void_p] + [_i64p] * 5
@@ -60,6 +69,8 @@ def lib():
         L.ref_get_tables.argtypes = [C.c_void_p] + [_f64p] * 8
         L.ref_cfl.restype = C.c_double
         L.ref_cfl.argtypes = [C.c_void_p]
+        L.ref_last_bad_index.restype = C.c_int64
+        L.ref_last_bad_index.argtypes = [C.c_void_p]
         L.ref_phys_deriv.restype = C.c_double
         L.ref_phys_deriv.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32]
         L.ref_initial_solution.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _f64p]
@@ -81,10 +92,8 @@ def lib():
         L.ref_flux.argtypes = [_f64p, C.c_double, _f64p, _f64p]
         L.ref_eigenmatrix.argtypes = [_f64p, C.c_double, C.c_double, C.c_double, _f64p, _f64p]
         L.ref_rk3_linear.argtypes = [_f64p, C.c_int64, C.c_double, C.c_double]
-        for f in ("ref_create", "ref_gmsh_parse"):
-            pass
-        _lib = L
-    return _lib
+        _libs[plant] = L
+    return _libs[plant]
 
 
 def _p(a, t):
@@ -112,8 +121,8 @@ class RefCase:
     """One oracle case: mesh + parameters, with the reference's solver operations."""
 
     def __init__(self, mesh, order=2, n_cell_gp=0, n_edge_gp=0, hcr=1.4, cfl=0.0, mach=0.8, aoa_deg=1.5, rho=1.0,
-                 p=1.0e5, fs_hcr=1.4, limiter=1, nthreads=1, gemm_fma=1):
-        L = lib()
+                 p=1.0e5, fs_hcr=1.4, limiter=1, nthreads=1, gemm_fma=1, plant=0):
+        L = lib(plant)
         m = mesh.contiguous()
         self.mesh = m
         self.prm = RefParams(order, n_cell_gp, n_edge_gp, hcr, cfl, mach, aoa_deg, rho, p, fs_hcr, limiter, nthreads,
@@ -250,6 +259,10 @@ class RefCase:
         s = self._L.ref_rk3(self.h, _p(R, _f64p), _p(U, _f64p), dt)
         return U, s
 
+    def last_bad_index(self):
+        """Edge at which the last failing residual stopped (the first failing interior edge in list order)."""
+        return int(self._L.ref_last_bad_index(self.h))
+
     def time_march(self, U, final_step, final_time):
         U = np.ascontiguousarray(U, np.float64).copy()
         R = self.zeros()
@@ -258,30 +271,30 @@ class RefCase:
         return U, s, n.value, t.value
 
 
-def hllc(ql, qr, n, hcr=1.4):
-    L = lib()
+def hllc(ql, qr, n, hcr=1.4, plant=0):
+    L = lib(plant)
     ql = np.ascontiguousarray(ql, np.float64); qr = np.ascontiguousarray(qr, np.float64)
     n = np.ascontiguousarray(n, np.float64); f = np.zeros(4)
     s = L.ref_hllc(_p(ql, _f64p), _p(qr, _f64p), _p(n, _f64p), hcr, _p(f, _f64p))
     return f, s
 
 
-def flux(q, hcr=1.4):
-    L = lib()
+def flux(q, hcr=1.4, plant=0):
+    L = lib(plant)
     q = np.ascontiguousarray(q, np.float64); f = np.zeros(4); g = np.zeros(4)
     L.ref_flux(_p(q, _f64p), hcr, _p(f, _f64p), _p(g, _f64p))
     return f, g
 
 
-def eigenmatrix(q, nx, ny, hcr=1.4):
-    L = lib()
+def eigenmatrix(q, nx, ny, hcr=1.4, plant=0):
+    L = lib(plant)
     q = np.ascontiguousarray(q, np.float64); Lm = np.zeros(16); Rm = np.zeros(16)
     L.ref_eigenmatrix(_p(q, _f64p), nx, ny, hcr, _p(Lm, _f64p), _p(Rm, _f64p))
     return Lm.reshape(4, 4), Rm.reshape(4, 4)
 
 
-def rk3_linear(u, lam, dt):
-    L = lib()
+def rk3_linear(u, lam, dt, plant=0):
+    L = lib(plant)
     u = np.ascontiguousarray(u, np.float64).copy()
     L.ref_rk3_linear(_p(u, _f64p), u.size, lam, dt)
     return u

@@ -109,7 +109,8 @@ def test_sequential_walk_matches_level_launches(all_meshes, name, precision, mon
         nbr = ctx.connectivity()["ineighbours"]
     (Vw, tw), (Vl, tl), (Vj, _) = out["walk"], out["levels"], out["jacobi"]
     assert np.array_equal(tw, tl)
-    assert rel_err(Vw, Vl) <= (TOL64 if precision == "fp64" else 1e-6), rel_err(Vw, Vl)
+    # the walk and the per-level launches run the same detector and rebuild code on the same inputs
+    assert np.array_equal(Vw, Vl), rel_err(Vw, Vl)
     chained = [e for e in np.nonzero(tw)[0] if any(0 <= j < e and tw[j] for j in nbr[e])]
     assert len(chained) > 0, "some troubled cell must follow a troubled lower-numbered neighbour"
     assert rel_err(Vw, Vj) > 1e-9, "the element order must matter on this state"
