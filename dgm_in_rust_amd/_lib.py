@@ -21,7 +21,7 @@ LIMITER_OFF, LIMITER_SEQUENTIAL, LIMITER_JACOBI = 0, 1, 2
 
 EXPORTED = ["dgm_ctx_create", "dgm_ctx_create_host", "dgm_ctx_destroy", "dgm_ctx_info", "dgm_last_error", "dgm_get_connectivity",
             "dgm_get_geometry", "dgm_get_tables", "dgm_initial_solution", "dgm_project", "dgm_cell_points_xy",
-            "dgm_compute_residuals", "dgm_apply_limiter", "dgm_compute_time_step", "dgm_rk3", "dgm_time_march",
+            "dgm_compute_residuals", "dgm_compute_residuals_device_noalloc", "dgm_apply_limiter", "dgm_compute_time_step", "dgm_rk3", "dgm_time_march",
             "dgm_state_upload", "dgm_state_download", "dgm_device_time_march", "dgm_device_steps_async",
             "dgm_synchronize", "dgm_stream", "dgm_check_state", "dgm_device_clock", "dgm_lift_coefficient",
             "dgm_profile_steps", "dgm_troubled_counts", "dgm_get_tuning", "dgm_set_tuning", "dgm_autotune",
@@ -125,6 +125,7 @@ def lib():
     L.dgm_project.argtypes = [vp, _f64p, _f64p]
     L.dgm_cell_points_xy.argtypes = [vp, _f64p]
     L.dgm_compute_residuals.argtypes = [vp, _f64p, _f64p, C.c_int32]
+    L.dgm_compute_residuals_device_noalloc.argtypes = [vp, vp, vp]   # device pointers
     L.dgm_apply_limiter.argtypes = [vp, _f64p, _u8p]
     L.dgm_compute_time_step.argtypes = [vp, _f64p, _f64p]
     L.dgm_rk3.argtypes = [vp, _f64p, C.c_double]

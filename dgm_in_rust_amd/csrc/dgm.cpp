@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +17,14 @@
 #include "solver.h"
 
 using namespace dgm;
+
+// Working memory of the allocation-free residual (dgm_compute_residuals_device_noalloc): the
+// structure-of-arrays state and residual and the edge point fluxes of one call. Every GPU the library
+// runs on holds its own copy until the process ends; calls that overlap on one GPU share it and race.
+// Host code never uses this name as an address: each device's address comes from
+// hipGetSymbolAddress (scratch_address below) and is passed to the kernels as a plain pointer.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 struct dgm_ctx {
   dgm_params prm{};
@@ -35,6 +44,8 @@ struct dgm_ctx {
   hipGraphExec_t graph_exec = nullptr;
   dgm_error err{};
   double cfl = 0.0;
+  void* scratch = nullptr;   // this device's address of the scratch array
+  CaseCtl* h_ctl = nullptr;  // pinned host copy of the control record (read-back without allocation)
 };
 
 namespace {
@@ -70,6 +81,30 @@ int dalloc(dgm_ctx* c, X** p, size_t n) {
 size_t nstate(const dgm_ctx* c) { return (size_t)c->M.ne * 4 * c->T.np; }
 DeviceState& dev(dgm_ctx* c) { return c->A->S; }
 const Solver& solver(dgm_ctx* c) { return *c->A->solver; }
+
+// Device address of the scratch array on the current device, looked up once per device.
+int scratch_address(void** out, std::string& err) {
+  static std::mutex mu;
+  static void* addr[256] = {};
+  int d = 0;
+  hipError_t e = hipGetDevice(&d);
+  if (e != hipSuccess || d < 0 || d >= 256) {
+    err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+    return DGM_HIP_ERROR;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[d]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return DGM_HIP_ERROR;
+    }
+    addr[d] = p;
+  }
+  *out = addr[d];
+  return DGM_OK;
+}
 
 int read_ctl(dgm_ctx* c, CaseCtl& h) {
   if (!c->stream) return fail(c, DGM_BAD_ARGUMENT, "host-only context");
@@ -167,6 +202,8 @@ int build_device(dgm_ctx* c) {
   if (int s = dalloc(c, &c->d_dtmin, DT_CASE)) return s;
   if (int s = dalloc(c, &c->d_force, 2)) return s;
   if (int s = dalloc(c, &c->d_flag, 1)) return s;
+  if (int s = scratch_address(&c->scratch, e)) return fail(c, s, e);
+  HIPCHK(c, hipHostMalloc((void**)&c->h_ctl, sizeof(CaseCtl), hipHostMallocDefault));
   c->A.reset(new Arena());
   Arena& A = *c->A;
   if (int s = A.init(P.order, P.n_cell_gp, P.n_edge_gp, P.precision, P.limiter, c->M.ne, c->M.nedge,
@@ -266,6 +303,7 @@ void dgm_ctx_destroy(dgm_ctx* c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   c->A.reset();
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->h_ctl) (void)hipHostFree(c->h_ctl);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -428,6 +466,42 @@ int dgm_compute_residuals(dgm_ctx* c, double* R, const double* U, int32_t accumu
         }
   }
   return DGM_OK;
+}
+
+int dgm_compute_residuals_device_noalloc(dgm_ctx* c, const double* U, double* R) {
+  if (!c || !U || !R) return DGM_BAD_ARGUMENT;
+  if (!c->stream || !c->scratch || !c->h_ctl) return fail(c, DGM_BAD_ARGUMENT, "host-only context");
+  // the call's working arrays, carved from the scratch array at 256-byte boundaries
+  DeviceState S = dev(c);
+  const bool f32 = solver(c).fp32();
+  const size_t ns = (size_t)4 * c->T.np * (size_t)S.mesh.nes;
+  const size_t nflux = (size_t)S.mesh.ed_stride * c->T.ne * 4;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = (off + bytes + 255) / 256 * 256;
+    return static_cast<unsigned char*>(c->scratch) + at;
+  };
+  S.un = reinterpret_cast<double*>(take(ns * 8));
+  S.rout = reinterpret_cast<double*>(take(ns * 8));
+  S.sn = f32 ? take(ns * 4) : nullptr;
+  S.flux = take(nflux * (f32 ? 4 : 8));
+  if (off > kScratchBytes)
+    return fail(c, DGM_NO_SPACE,
+                "dgm_compute_residuals_device_noalloc: this mesh and order need " + std::to_string(off) +
+                    " bytes of working memory; the scratch array holds " + std::to_string(kScratchBytes));
+  S.prof_ev[0] = S.prof_ev[1] = nullptr;
+  // Nothing in the scratch must start at zero: the state and the point fluxes are written in full
+  // before they are read, and the residual kernel writes every element's row. The control record
+  // (the error word the edge kernel reports into) is reset here by a kernel, not by a host copy.
+  launch_set_case(c->d_case, fresh_ctl(c), c->stream);
+  solver(c).upload(S, U, 0, (int)c->M.ne, c->stream);
+  solver(c).residual(S, c->stream);
+  solver(c).download(S, S.rout, R, 0, (int)c->M.ne, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->h_ctl, c->d_case, sizeof(CaseCtl), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ctl_status(c, *c->h_ctl);
 }
 
 int dgm_apply_limiter(dgm_ctx* c, double* U, uint8_t* troubled) {

@@ -992,6 +992,9 @@ __global__ __launch_bounds__(64) void k_step_begin(Ctl ctl, int ncases) {
 
 __global__ void k_reset_count(Ctl ctl, int slot) { ctl.count[slot] = 0; }
 
+// writes a whole control record from a kernel argument (no host-to-device copy)
+__global__ void k_set_case(CaseCtl* dst, CaseCtl value) { *dst = value; }
+
 // ---------------------------------------------------------------------------------------
 // limiter
 // ---------------------------------------------------------------------------------------
@@ -1833,6 +1836,8 @@ void go(const DeviceState& s, void (*kernel)(KArgs...), dim3 grid, dim3 block, h
 void launch_step_begin(Ctl ctl, int ncases, hipStream_t st) {
   if (ncases > 0) k_step_begin<<<ncases, 64, 0, st>>>(ctl, ncases);
 }
+
+void launch_set_case(CaseCtl* dst, const CaseCtl& value, hipStream_t st) { k_set_case<<<1, 1, 0, st>>>(dst, value); }
 
 // ---------------------------------------------------------------------------------------
 // Launch sequences for one (precision, order, rule) combination

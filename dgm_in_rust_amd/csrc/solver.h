@@ -82,5 +82,7 @@ std::unique_ptr<Solver> make_solver(int precision, int p, int nq, int ne);
 
 // per-case step control: one thread per case slot
 void launch_step_begin(Ctl ctl, int ncases, hipStream_t st);
+// stream-ordered write of one control record by a one-thread kernel
+void launch_set_case(CaseCtl* dst, const CaseCtl& value, hipStream_t st);
 
 }  // namespace dgm

@@ -148,6 +148,16 @@ int dgm_rk3(dgm_ctx* ctx, double* solutions, double dt);
 int dgm_time_march(dgm_ctx* ctx, double* solutions, int32_t final_step, double final_time, dgm_march_stats* stats);
 
 /* ---- device-resident state ---- */
+/* Allocation-free residual (extension; no counterpart in the reference). For callers that cannot
+ * allocate device memory: harnesses that forbid hipMalloc, tight serving loops. Same result as
+ * dgm_compute_residuals with accumulate = 0 (overwrite), bit for bit, but U and R are DEVICE arrays in
+ * the reference layout [n_elements][4][n_basis] f64, and the call allocates nothing: its working arrays
+ * (the structure-of-arrays state and residual, the edge point fluxes) live in a fixed 128 MiB scratch
+ * array that the library holds on each GPU until the process ends. A context whose working set exceeds
+ * it (about 150k triangles at p = 3 in FP64) is refused with DGM_NO_SPACE. Runs on the context stream
+ * and returns when the result is in R. Calls that overlap on one GPU (contexts used from several
+ * threads at once) share the scratch array and race. */
+int dgm_compute_residuals_device_noalloc(dgm_ctx* ctx, const double* solutions_dev, double* residuals_dev);
 int dgm_state_upload(dgm_ctx* ctx, const double* solutions);
 int dgm_state_download(dgm_ctx* ctx, double* solutions);
 /* time_march on the resident state: steps run as replays of a captured HIP graph of one RK3 step. */
