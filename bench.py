@@ -11,12 +11,22 @@ weights and positivity tests); the plain FP64 path is measured in the same run.
 
 GDOF * stage / s = n_elements * 4 * n_basis * 3 * steps / seconds / 1e9.
 
-The roofline object is for the dominant kernel, with its algorithmic bytes per launch and its
-mean launch time from HIP events on the context stream (dgm_profile_steps); traffic is the HBM
-bytes per dispatch that rocprofv3's FETCH_SIZE / WRITE_SIZE counters measured for the same kernel,
-read from the workload's committed summary under profiles/ (null when there is none).
-The CPU baseline times the oracle (FP64 C++ restatement of the reference, OpenMP in the three
-residual folds like the reference's rayon folds) on a few steps of the same case.
+The roofline object is for the whole step, on the algorithmic bytes of one RK stage (every array a stage
+must touch, once: B = 80 Np + 53 bytes per element-stage in FP32-bulk, 117.33 Np + 91 in FP64) times the
+elements and the 3 stages, over the measured time per step; its traffic is the HBM bytes one stage's
+kernels moved, summed from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed for this workload under
+profiles/ (null when there is none). Its dominant_kernel sub-object puts the element pass alone against
+the same per-stage bytes, with its launch time from HIP events on the context stream (dgm_profile_steps)
+and its own counter traffic; kernel_accounting gives each pass's own byte count. The mfma object is the
+matrix-core element pass's issued and useful FP32 flops against the 157.3 TFLOP/s dense FP32 matrix peak.
+The CPU baseline times the oracle (FP64 C++ restatement of the reference, OpenMP in the three residual
+folds like the reference's rayon folds, sequential element-order WENO like the reference) on the host:
+20 steps of the same case on one thread, 5 on the host-core share of one GPU (the faster is the
+headline), and the full 500 steps of the 16k-triangle p = 2 configuration.
+
+A second FP32-bulk timing starts from a state with a standing normal shock near the airfoil
+(cases.shocked_start), so the limiter captures a shock from the first step; both report the fraction of
+cells the limiter rebuilt.
 
 Also measured in the same run (sub-objects of the JSON line):
 * sweep: 256 airfoil cases (log-uniform 2k-60k triangles, p in {1, 2, 3}, M in [0.3, 0.85], AoA in
@@ -70,10 +80,13 @@ def stage_bytes_min(np_, fp32):
     return (80.0 * np_ + 53.0) if fp32 else (117.33 * np_ + 91.0)
 
 
-def run_gpu(mesh, precision, steps, warmup, order, mach, aoa, tuning=None):
-    from dgm_in_rust_amd import Context
+def run_gpu(mesh, precision, steps, warmup, order, mach, aoa, tuning=None, start="freestream"):
+    from dgm_in_rust_amd import Context, cases
     ctx = Context(mesh, order=order, precision=precision, limiter="jacobi", mach=mach, aoa_deg=aoa)
-    U0 = ctx.initial_solution(1.0, aoa, mach, 1.0e5)
+    if start == "shock":
+        U0 = cases.shocked_start(ctx, mach, aoa)
+    else:
+        U0 = ctx.initial_solution(1.0, aoa, mach, 1.0e5)
     ctx.upload(U0)
     if warmup:
         ctx.steps_async(warmup)
@@ -93,13 +106,18 @@ def run_gpu(mesh, precision, steps, warmup, order, mach, aoa, tuning=None):
     ctx.check_state()
     kt = ctx.profile_steps(20)
     info = ctx.info
-    out = dict(ctx=ctx, wall=wall, steps_done=nsteps, t=tnow, dt=dt, kt=kt, tuning=tuning, ne=info.n_elements,
+    out = dict(ctx=ctx, wall=wall, steps_done=nsteps, t=tnow, dt=dt, kt=kt, tuning=tuning, ne=info.n_elements, order=order,
                nedge=info.n_edges,
                np=info.n_basis, nq=info.n_cell_gp, ng=info.n_edge_gp, cfl=info.cfl, troubled=troubled)
     return out
 
 
+MFMA_FP32_PEAK = 157.3   # TFLOP/s, dense FP32 matrix peak of the MI355X
+
+
 def summarise(r, steps, fp32):
+    """Throughput; the step roofline on the per-stage algorithmic bytes; the element pass against the same
+    bytes and against its own byte count; matrix-core utilisation of the FP32-bulk element pass."""
     dof = r["ne"] * 4 * r["np"]
     value = dof * 3 * steps / r["wall"] / 1e9
     tsize = 4 if fp32 else 8
@@ -108,34 +126,55 @@ def summarise(r, steps, fp32):
     per = {"element": kt["element_ms"], "surface": kt["surface_ms"], "detect": kt["detect_ms"],
            "rebuild": kt["rebuild_ms"], "apply": kt["apply_ms"], "time_step": kt["time_step_ms"],
            "step_control": kt["control_ms"]}
-    dom = max(("element", "surface"), key=lambda k: per[k])
-    ach = kb[dom] / (per[dom] * 1e-3) / 1e9
-    stage_ms = 1e3 * r["wall"] / steps / 3.0
-    sb = stage_bytes_min(r["np"], fp32) * r["ne"]
-    return dict(value=value, ms_per_step=1e3 * r["wall"] / steps, dominant=dom, kernel_ms=per, tuning=r["tuning"],
+    ms_step = 1e3 * r["wall"] / steps
+    stage_b = stage_bytes_min(r["np"], fp32) * r["ne"]
+    step_gbs = 3.0 * stage_b / (ms_step * 1e-3) / 1e9
+    mfma_kernel = fp32 and r["np"] <= 16 and r["nq"] <= 16 and 3 * r["ng"] <= 16 and r["tuning"].get("element_waves", 0) > 0
+    dom = "element"
+    dom_name = ("k_element_mfma" if mfma_kernel else "k_element<float>") if fp32 else "k_element<double>"
+    dom_s = per[dom] * 1e-3
+    roof = dict(bound="hbm", achieved=step_gbs, peak=HBM_PEAK_GBS, unit="GB/s", frac=step_gbs / HBM_PEAK_GBS,
+                traffic=None,
+                basis=f"{stage_bytes_min(r['np'], fp32):.1f} algorithmic bytes per element-stage x {r['ne']} elements "
+                      f"x 3 stages per step / {ms_step:.4f} ms per step",
+                dominant_kernel=dict(name=dom_name, avg_us=per[dom] * 1e3, bytes=stage_b,
+                                     achieved=stage_b / dom_s / 1e9, frac=stage_b / dom_s / 1e9 / HBM_PEAK_GBS,
+                                     basis="the per-stage algorithmic bytes over the element pass's mean launch time"),
+                kernel_accounting={k: dict(bytes=kb[k], avg_us=per[k] * 1e3, achieved=kb[k] / (per[k] * 1e-3) / 1e9,
+                                           frac=kb[k] / (per[k] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                                   for k in ("element", "surface") if per[k] > 0})
+    mfma = None
+    if mfma_kernel:
+        # 64 v_mfma_f32_16x16x4f32 (2048 flops each) per 16 elements: values 16, volume 32, lift 16; useful:
+        # values 2*4*Nq*Np, volume 2*8*Nq*Np, lift 2*4*3Ne*Np per element
+        issued = r["ne"] / 16.0 * 64 * 2048
+        useful = r["ne"] * (2 * 4 * r["nq"] * r["np"] + 2 * 8 * r["nq"] * r["np"] + 2 * 4 * 3 * r["ng"] * r["np"])
+        mfma = dict(kernel=dom_name, issued_tflops=issued / dom_s / 1e12, useful_tflops=useful / dom_s / 1e12,
+                    peak_tflops=MFMA_FP32_PEAK, issued_frac=issued / dom_s / 1e12 / MFMA_FP32_PEAK,
+                    useful_frac=useful / dom_s / 1e12 / MFMA_FP32_PEAK, gflop_issued_per_launch=issued / 1e9,
+                    gflop_useful_per_launch=useful / 1e9)
+    return dict(value=value, ms_per_step=ms_step, kernel_ms=per, tuning=r["tuning"],
                 troubled_fraction=sum(r["troubled"]) / (3.0 * r["ne"]), sim_time=r["t"], last_dt=r["dt"],
-                roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                              traffic=None, kernel=(("k_element_mfma" if fp32 else "k_element") if dom == "element" else "k_surface"),
-                              bytes_per_launch=kb[dom]),
-                stage=dict(bytes_per_element_stage=stage_bytes_min(r["np"], fp32), ms=stage_ms,
-                           achieved=sb / (stage_ms * 1e-3) / 1e9, frac=sb / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS))
+                roofline=roof, mfma=mfma)
 
 
 def attach_traffic(summ, r, args, fp32):
-    """roofline.traffic from the committed rocprofv3 counter summary of this workload, when there is one:
-    HBM bytes per dispatch of the dominant kernel (mean over its three stage variants), with the
-    profile's mean duration beside the HIP-event one for comparison."""
+    """Counter traffic from the committed rocprofv3 summary of this workload, when there is one: HBM bytes
+    per stage (the stage's edge, element, detector and rebuild passes, each the mean over its launches)
+    for the step roofline, and per launch of the element pass for its sub-object."""
     from dgm_in_rust_amd.rocprof_summary import read_traffic
     name = f"kernels_naca0012_{args.surface}x{args.layers}_p{args.order}.txt"
+    path = os.path.join(ROOT, "profiles", name)
     dims = f"{r['np']},{r['nq']},{r['ng']}"
-    if summ["dominant"] == "element":
-        prefix = f"k_element_mfma<{dims}," if fp32 else f"k_element<double,{dims},"
-    else:
-        prefix = f"k_surface<{'float' if fp32 else 'double'},{dims}>"
-    got = read_traffic(os.path.join(ROOT, "profiles", name), prefix)
-    if got:
-        summ["roofline"]["traffic"] = got[0]
-        summ["roofline"]["profile_avg_us"] = got[1]
+    t = "float" if fp32 else "double"
+    elem = f"k_element_mfma<{dims}," if summ["roofline"]["dominant_kernel"]["name"] == "k_element_mfma" else f"k_element<{t},{dims},"
+    parts = [elem, f"k_surface<{t},{dims}>", f"k_kxrcf<{t},{r['order']},", f"k_weno<{t},{r['order']},"]
+    got = [read_traffic(path, pre) for pre in parts]
+    if got[0]:
+        summ["roofline"]["dominant_kernel"]["traffic"] = got[0][0]
+        summ["roofline"]["dominant_kernel"]["profile_avg_us"] = got[0][1]
+    if all(got):
+        summ["roofline"]["traffic"] = sum(g[0] for g in got)
         summ["roofline"]["traffic_source"] = f"profiles/{name}"
 
 
@@ -192,18 +231,52 @@ def serve(rate, nreq, precision, max_steps=20000, cl_tol=1e-5, seed=0, time_limi
     return out
 
 
-def cpu_baseline(mesh, order, mach, aoa, nsteps):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(mesh, order, mach, aoa, nsteps, nsteps_1t, config1_steps):
+    """The oracle on the host: the same 128k-triangle p = 3 case from freestream for nsteps on the host-core
+    share of one GPU (OMP_NUM_THREADS, else every core), nsteps_1t on one thread, and config 1 (160 x 50
+    O-grid, p = 2, M = 0.5, CFL 0.3, 500 steps) on the same threads. Sequential element-order WENO, as the
+    reference; threads only in the three residual folds, as the reference's rayon folds."""
+    from dgm_in_rust_amd import meshgen
     from oracle.refcpu import RefCase
-    threads = max(1, min(16, os.cpu_count() or 1))
-    ref = RefCase(mesh, order=order, limiter=1, mach=mach, aoa_deg=aoa, nthreads=threads)
-    U = ref.initial_solution(1.0, aoa, mach, 1.0e5)
-    t0 = time.perf_counter()
-    U, s, n, t = ref.time_march(U, nsteps, 1.0e9)
-    sec = time.perf_counter() - t0
-    val = ref.ne * 4 * ref.nbasis * 3 * n / sec / 1e9
-    return dict(value=val, unit="GDOF·stage/s", cores=threads, kind="port",
-                sample=f"{n} SSP-RK3 steps of the same 128k-triangle p=3 case from freestream (oracle, FP64, "
-                       f"sequential WENO; {threads} OpenMP threads in the residual folds, {sec:.1f} s)")
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1)
+
+    def rate(m, p, M, a, n, nt, cfl=0.0):
+        ref = RefCase(m, order=p, limiter=1, mach=M, aoa_deg=a, nthreads=nt, cfl=cfl)
+        U = ref.initial_solution(1.0, a, M, 1.0e5)
+        t0 = time.perf_counter()
+        U, s, done, t = ref.time_march(U, n, 1.0e9)
+        sec = time.perf_counter() - t0
+        return ref.ne * 4 * ref.nbasis * 3 * done / sec / 1e9, sec, done
+
+    v1, sec1, done1 = rate(mesh, order, mach, aoa, nsteps_1t, 1)
+    v, sec, done = rate(mesh, order, mach, aoa, nsteps, threads)
+    # The headline is the faster of the two. The reference's residual folds give every rayon piece a
+    # zeroed full-size array and then add the pieces, and the oracle does the same per thread. So more
+    # threads can be slower than one.
+    best = (v1, 1, done1, sec1) if v1 >= v else (v, threads, done, sec)
+    out = dict(value=best[0], unit="GDOF·stage/s", cores=best[1], kind="port",
+               sample=f"{best[2]} SSP-RK3 steps of the same 128k-triangle p=3 case from freestream (oracle, FP64, "
+                      f"sequential element-order WENO as the reference; {best[1]} thread(s), OpenMP in the residual "
+                      f"folds only), {best[3]:.1f} s",
+               one_thread=dict(value=v1, steps=done1, seconds=sec1),
+               all_threads=dict(value=v, threads=threads, steps=done, seconds=sec), cpu_model=cpu_model(),
+               nproc=os.cpu_count())
+    if config1_steps > 0:
+        c1 = meshgen.naca0012_ogrid(160, 50, seed=0)
+        vc, secc, donec = rate(c1, 2, 0.5, 1.25, config1_steps, threads, cfl=0.3)
+        out["config1"] = dict(workload="naca0012_ogrid_160x50_p2_M0.5_aoa1.25_cfl0.3", steps=donec, seconds=secc,
+                              value=vc, threads=threads)
+    return out
 
 
 def main():
@@ -214,9 +287,12 @@ def main():
     ap.add_argument("--surface", type=int, default=640)
     ap.add_argument("--layers", type=int, default=100)
     ap.add_argument("--order", type=int, default=3)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps of the same case on the host-core share")
+    ap.add_argument("--cpu-steps-1t", type=int, default=20, help="oracle steps of the same case on one thread")
+    ap.add_argument("--config1-steps", type=int, default=500, help="oracle steps of the 16k-triangle p = 2 case (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-shock", action="store_true", help="skip the shocked-start FP32-bulk timing")
     ap.add_argument("--sweep-cases", type=int, default=256, help="batched sweep size (0: skip)")
     ap.add_argument("--sweep-steps", type=int, default=1000)
     ap.add_argument("--serve-requests", default="16,32,96",
@@ -240,7 +316,7 @@ def main():
             bb.copy_(a)
         torch.cuda.synchronize()
         print(f"copy probe: 5 x {args.copy_probe} MiB device copies", file=sys.stderr, flush=True)
-    from dgm_in_rust_amd import meshgen
+    from dgm_in_rust_amd import cases, meshgen
     mach, aoa = 0.8, 1.25
     mesh = meshgen.naca0012_ogrid(args.surface, args.layers, seed=0)
     tun = json.loads(args.tuning) if args.tuning is not None else None
@@ -258,10 +334,25 @@ def main():
     rel = None
     if s64:
         attach_traffic(s64, r64, args, False)
-        # both runs took warmup + steps + 20 profiled steps from the same state: compare them
-        from dgm_in_rust_amd import cases
+        # both runs took warmup + steps + 20 profiled steps from the same state: compare them per variable
         if r32["steps_done"] == r64["steps_done"]:
-            rel = cases.relative_l2(r64["ctx"], r32["ctx"].download(), r64["ctx"].download())
+            u32, u64 = r32["ctx"].download(), r64["ctx"].download()
+            rel = dict(per_variable=cases.relative_l2_per_variable(r64["ctx"], u32, u64),
+                       per_variable_own_norm=cases.relative_l2_per_variable(r64["ctx"], u32, u64, own_norm=True),
+                       pooled=cases.relative_l2(r64["ctx"], u32, u64))
+        r64["ctx"].close()
+    r32["ctx"].close()
+    shock = None
+    if not args.no_shock:
+        rs = run_gpu(mesh, "fp32", args.steps, args.warmup, args.order, mach, aoa, tun, start="shock")
+        ss = summarise(rs, args.steps, True)
+        shock = dict(value=ss["value"], ms_per_step=ss["ms_per_step"], troubled_fraction=ss["troubled_fraction"],
+                     kernel_ms=ss["kernel_ms"], roofline_frac=ss["roofline"]["frac"],
+                     start="freestream with a standing normal shock (M 1.3 Rankine-Hugoniot jump at x = 0.6 chord, "
+                           "|y| < 1.5 chords), projected; timed after the warm-up steps")
+        rs["ctx"].close()
+        print(f"fp32-bulk, shocked start: {ss['value']:.2f} GDOF*stage/s, troubled fraction "
+              f"{ss['troubled_fraction']:.4f}, kernels {ss['kernel_ms']}", file=sys.stderr, flush=True)
     sw = None
     if args.sweep_cases > 0:
         sw = sweep(args.sweep_cases, args.sweep_steps)
@@ -281,8 +372,9 @@ def main():
         print(f"serve fp64 rate {rates[mid]:g}: {sv['fp64'][f'{rates[mid]:g}']}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(mesh, args.order, mach, aoa, args.cpu_steps)
-        print(f"cpu baseline: {cpu['value']:.5f} GDOF*stage/s on {cpu['cores']} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(mesh, args.order, mach, aoa, args.cpu_steps, args.cpu_steps_1t, args.config1_steps)
+        print(f"cpu baseline: {cpu['all_threads']['value']:.5f} GDOF*stage/s on {cpu['all_threads']['threads']} threads, "
+              f"{cpu['one_thread']['value']:.5f} on one", file=sys.stderr, flush=True)
     line = {
         "metric": METRIC, "value": s32["value"], "unit": "GDOF·stage/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": s32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -293,18 +385,18 @@ def main():
                    "precision": "fp32_bulk (fp64 u^n, RK accumulation, WENO weights, positivity)",
                    "parallelism": "single_gpu", "start": "freestream"},
         "roofline": s32["roofline"],
-        "stage_roofline": s32["stage"],
+        "mfma": s32["mfma"],
         "kernel_ms": s32["kernel_ms"],
         "tuning": s32["tuning"],
         "troubled_fraction": s32["troubled_fraction"],
+        "shocked_start": shock,
         "cpu_baseline": cpu,
         "sweep": sw,
         "serving": sv,
     }
     if s64:
         line["fp64"] = {"value": s64["value"], "ms_per_step": s64["ms_per_step"], "roofline": s64["roofline"],
-                        "stage_roofline": s64["stage"], "kernel_ms": s64["kernel_ms"],
-                        "troubled_fraction": s64["troubled_fraction"]}
+                        "kernel_ms": s64["kernel_ms"], "troubled_fraction": s64["troubled_fraction"]}
         line["fp32_over_fp64"] = s32["value"] / s64["value"]
         line["fp32_vs_fp64_rel_l2"] = rel
         line["fp32_vs_fp64_steps"] = r32["steps_done"]

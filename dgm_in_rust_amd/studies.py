@@ -2,7 +2,7 @@
 [0, 10]^2 square): N x N jittered quads (+-10 %) split by random diagonals into 2 N^2 triangles,
 Dubiner order p, limiter off, SSP-RK3 at the reference CFL to t_end. For every (N, p) it reports the
 L2 error against the exact solution in FP64 and in FP32-bulk, the relative L2 difference between the
-two, the observed convergence order between successive N, the step count and the wall time.
+two per conserved variable, the observed convergence order between successive N, the step count and the wall time.
 
 python -m dgm_in_rust_amd.studies [--sizes 64,128,256,512] [--orders 1,2,3,4] [--t-end 2.0]
 prints one JSON object per run, then a markdown table.
@@ -116,18 +116,21 @@ def main(argv=None):
             c64, U64, r64 = vortex_run(n, p, "fp64", a.t_end)
             c32, U32, r32 = vortex_run(n, p, "fp32", a.t_end)
             row = dict(r64)
+            pv = cases.relative_l2_per_variable(c64, U32, U64)
             row.update(l2_density_fp32=r32["l2_density"], l2_all_fp32=r32["l2_all"], wall_fp32_s=r32["wall_s"],
-                       steps_fp32=r32["steps"], fp32_vs_fp64_rel_l2=cases.relative_l2(c64, U32, U64))
+                       steps_fp32=r32["steps"], fp32_vs_fp64_rel_l2=cases.relative_l2(c64, U32, U64),
+                       fp32_vs_fp64_per_variable=pv, fp32_vs_fp64_max_variable=max(pv.values()))
             row["rate"] = None if prev is None else math.log2(prev / row["l2_density"])
             prev = row["l2_density"]
             rows.append(row)
             print(json.dumps(row), flush=True)
-    print("\n| N | p | triangles | steps | L2 err (rho) FP64 | L2 err (rho) FP32-bulk | rate | FP32 vs FP64 rel. L2 | wall FP64 / FP32 (s) |")
+    print("\n| N | p | triangles | steps | L2 err (rho) FP64 | L2 err (rho) FP32-bulk | rate | FP32 vs FP64 rel. L2: rho, rho u, rho v, E | wall FP64 / FP32 (s) |")
     print("|---|---|---|---|---|---|---|---|---|")
     for r in rows:
         rate = "" if r["rate"] is None else f"{r['rate']:.2f}"
+        pv = ", ".join(f"{v:.1e}" for v in r["fp32_vs_fp64_per_variable"].values())
         print(f"| {r['n']} | {r['order']} | {r['triangles']} | {r['steps']} | {r['l2_density']:.3e} | "
-              f"{r['l2_density_fp32']:.3e} | {rate} | {r['fp32_vs_fp64_rel_l2']:.2e} | {r['wall_s']:.1f} / {r['wall_fp32_s']:.1f} |")
+              f"{r['l2_density_fp32']:.3e} | {rate} | {pv} | {r['wall_s']:.1f} / {r['wall_fp32_s']:.1f} |")
 
 
 if __name__ == "__main__":

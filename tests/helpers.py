@@ -48,3 +48,15 @@ def rel_err(a, b):
 
 def rel_l2(a, b):
     return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) / np.linalg.norm(np.asarray(b).ravel()))
+
+
+def rel_l2_per_var(a, b):
+    """[4] relative L2 difference of each conserved variable (variables on axis 1), coefficient norm: rho and
+    E against their own norms, each momentum component against the norm of the momentum vector (a
+    component's own norm vanishes with the flow angle, and the error it carries does not)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    num = np.sqrt(((a - b) ** 2).sum(axis=(0, 2)))
+    own = np.sqrt((b ** 2).sum(axis=(0, 2)))
+    den = np.array([own[0], np.hypot(own[1], own[2]), np.hypot(own[1], own[2]), own[3]])
+    return num / np.maximum(den, 1e-300)

@@ -11,12 +11,17 @@ import pytest
 from dgm_in_rust_amd import Context, SpatialDisc, TemperalDisc, meshgen
 from dgm_in_rust_amd._lib import DgmError
 from oracle.refcpu import RefCase
-from tests.helpers import flow_state, rel_err, rel_l2
+from tests.helpers import flow_state, rel_err, rel_l2, rel_l2_per_var
 
 pytestmark = pytest.mark.gpu
 
 LIM = {"off": 0, "sequential": 1, "jacobi": 2}
 TOL64 = 1e-12
+# FP32-bulk against the FP64 oracle, per conserved variable (momentum components against the momentum
+# norm): the residual of one state (measured up to 2.0e-5, on the O-grid's transverse momentum), and
+# a 10-step march with WENO active (measured up to 6.1e-7)
+FP32_RESID_TOL = 4e-5
+FP32_WENO_TOL = 1e-6
 
 
 def pair(mesh, order, limiter="sequential", precision="fp64", **kw):
@@ -215,8 +220,10 @@ def test_residual_fp32_bulk(all_meshes, name, variant):
     U = flow_state(ref, seed=17, shock=True, mach=kw["mach"])
     Rref, s, _ = ref.compute_residuals(U)
     R = SpatialDisc(ctx).residuals(U)
-    # FP32 arithmetic on O(1e5) energies: compare against the residual scale
-    assert rel_l2(R, Rref) <= 2e-5, rel_l2(R, Rref)
+    # FP32 arithmetic on the state: each conserved variable's residual against its own scale
+    err = rel_l2_per_var(R, Rref)
+    print(f"{name} {variant}: residual rel L2 per variable {err}", flush=True)
+    assert err.max() <= FP32_RESID_TOL, err
 
 
 @pytest.mark.parametrize("name,cfl", [("ogrid_p3", 0.3), ("ogrid_p1", 0.0), ("periodic_p4", 0.0)])
@@ -229,7 +236,27 @@ def test_time_march_fp32_bulk(all_meshes, name, cfl):
     assert s == 0 and n == 20
     V = U0.copy()
     TemperalDisc(ctx, final_step=20, final_time=100.0).time_march(SpatialDisc(ctx), ctx.zeros(), V)
-    assert rel_l2(V, Uref) <= 1e-6, rel_l2(V, Uref)
+    err = rel_l2_per_var(V, Uref)
+    print(f"{name}: 20-step state rel L2 per variable {err}", flush=True)
+    assert err.max() <= 1e-6, err
+
+
+@pytest.mark.parametrize("name", ["ogrid_p3", "airfoil_p2"])
+def test_time_march_fp32_bulk_weno(all_meshes, name):
+    """FP32-bulk with the WENO limiter (Jacobi) against the FP64 oracle, from a shocked state whose
+    troubled cells are rebuilt every stage: 10 steps, per conserved variable."""
+    mesh, order, kw = all_meshes[name]
+    ref, ctx = pair(mesh, order, limiter="jacobi", precision="fp32", **kw)
+    U0 = flow_state(ref, seed=21, shock=True, mach=kw["mach"], amp=0.02)
+    ref_tr = ref.apply_limiter(U0)[1]
+    assert ref_tr.sum() > 0
+    Uref, s, n, t = ref.time_march(U0, 10, 100.0)
+    assert s == 0 and n == 10
+    V = U0.copy()
+    TemperalDisc(ctx, final_step=10, final_time=100.0).time_march(SpatialDisc(ctx), ctx.zeros(), V)
+    err = rel_l2_per_var(V, Uref)
+    print(f"{name}: 10-step WENO state rel L2 per variable {err}, {int(ref_tr.sum())} troubled", flush=True)
+    assert err.max() <= FP32_WENO_TOL, err
 
 
 def test_negative_density_reports_edge(all_meshes):
