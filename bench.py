@@ -32,10 +32,12 @@ Also measured in the same run (sub-objects of the JSON line):
 * sweep: 256 airfoil cases (log-uniform 2k-60k triangles, p in {1, 2, 3}, M in [0.3, 0.85], AoA in
   [-4, 6] deg) advanced together in one batch for 1000 steps each, FP32-bulk, autotuned launch shapes;
   aggregate GDOF * stage / s.
-* serving: Poisson arrivals at 2, 8 and 32 requests/s (16, 32 and 96 requests) of cases from the
-  same distribution, each run until its lift coefficient changes by less than 1e-5 over 100 steps or
-  for 20 000 steps; p50 / p99 time-to-converged-lift (arrival to exit), throughput and peak
-  concurrency, FP32-bulk at every rate and FP64 at the middle rate.
+* serving: Poisson arrivals at 2, 8 and 32 requests/s (64 requests each) of cases from the same
+  distribution, each run until its lift coefficient changes by less than 1e-5 over 100 steps or for
+  20 000 steps; p50 / p99 latency (arrival to exit) over all requests and per outcome (converged,
+  capped at 20 000 steps, failed), throughput, peak concurrency and the GPU-busy fraction (device
+  time of the step launches by HIP events / wall time), FP32-bulk at every rate and FP64 at the
+  middle rate.
 """
 import argparse
 import json
@@ -295,7 +297,7 @@ def main():
     ap.add_argument("--no-shock", action="store_true", help="skip the shocked-start FP32-bulk timing")
     ap.add_argument("--sweep-cases", type=int, default=256, help="batched sweep size (0: skip)")
     ap.add_argument("--sweep-steps", type=int, default=1000)
-    ap.add_argument("--serve-requests", default="16,32,96",
+    ap.add_argument("--serve-requests", default="64",
                     help="requests at each serving rate, comma-separated like --serve-rates (0: skip serving)")
     ap.add_argument("--serve-rates", default="2,8,32")
     ap.add_argument("--tuning", default=None,

@@ -97,6 +97,7 @@ class Batch:
         return i
 
     def remove(self, case_id: int):
+        """Retires a case (stream-ordered, without waiting for the queued steps)."""
         self._check(self._L.dgm_batch_remove(self.h, case_id), "dgm_batch_remove")
         self._case_order.pop(case_id, None)
         self._case_ne.pop(case_id, None)
@@ -118,6 +119,30 @@ class Batch:
         U = np.zeros((self._case_ne[case_id], 4, n_basis(self._case_order[case_id])))
         self._check(self._L.dgm_batch_case_state(self.h, case_id, ptr(U)), "dgm_batch_case_state")
         return U
+
+    # ---- serving polls ----
+    def poll_async(self, chord: float = 1.0, cl_tol: float = 1e-5) -> int:
+        """Queues a poll after the work already queued (see dgm_batch_poll_async); returns its ticket."""
+        t = C.c_int64(-1)
+        self._check(self._L.dgm_batch_poll_async(self.h, chord, cl_tol, C.byref(t)), "dgm_batch_poll_async")
+        return int(t.value)
+
+    def poll_wait(self, ticket: int, n: Optional[int] = None):
+        """Waits for the poll `ticket` (the oldest outstanding); returns (records as dicts, lift coefficients,
+        converged marks) of slots 0 .. n-1 as they were when the poll ran."""
+        n = self.max_cases if n is None else n
+        arr = (_lib.CaseInfo * n)()
+        cl = np.zeros(n)
+        conv = np.zeros(n, np.int32)
+        self._check(self._L.dgm_batch_poll_wait(self.h, ticket, n, arr, ptr(cl), ptr(conv, _lib._i32p)),
+                    "dgm_batch_poll_wait")
+        return [{k: getattr(a, k) for k, _ in _lib.CaseInfo._fields_} for a in arr], cl, conv.astype(bool)
+
+    def device_ms(self) -> float:
+        """Device time of the completed step launches so far, in ms (events around each steps_async call)."""
+        v = C.c_double(0.0)
+        self._check(self._L.dgm_batch_device_time(self.h, C.byref(v)), "dgm_batch_device_time")
+        return float(v.value)
 
     def lift(self, chord: float = 1.0, n: Optional[int] = None) -> np.ndarray:
         n = self.max_cases if n is None else n

@@ -397,7 +397,8 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
   return DGM_OK;
 }
 
-// Copies (dst, src, bytes) pieces through one pinned staging buffer on the arena's stream, then waits.
+// Copies (dst, src, bytes) pieces through one pinned staging buffer on the placement stream, then waits
+// for that stream.
 int Arena::stage_copy(const std::vector<std::pair<void*, std::pair<const void*, size_t>>>& pieces, std::string& err) {
   size_t total = 0;
   for (const auto& p : pieces) total += (p.second.second + 255) / 256 * 256;
@@ -412,16 +413,11 @@ int Arena::stage_copy(const std::vector<std::pair<void*, std::pair<const void*, 
     const size_t n = p.second.second;
     if (n == 0) continue;
     std::memcpy((char*)pinned_ + off, p.second.first, n);
-    ACHK(hipMemcpyAsync(p.first, (char*)pinned_ + off, n, hipMemcpyHostToDevice, stream));
+    ACHK(hipMemcpyAsync(p.first, (char*)pinned_ + off, n, hipMemcpyHostToDevice, pstream()));
     off += (n + 255) / 256 * 256;
   }
-  ACHK(hipStreamSynchronize(stream));
+  ACHK(hipStreamSynchronize(pstream()));
   return DGM_OK;
-}
-
-int Arena::set_case64(int* map, int64_t first, int64_t n, int value, std::string& err) {
-  std::vector<int> v((size_t)n, value);
-  return stage_copy({{map + first, {v.data(), v.size() * 4}}}, err);
 }
 
 int Arena::place(const HostMesh& M, const std::vector<double>& patch_q, int case_id, Placement& out, std::string& err) {
@@ -512,7 +508,7 @@ int Arena::place(const HostMesh& M, const std::vector<double>& patch_q, int case
   std::vector<int> elc((size_t)(pl.ne_alloc / 64), case_id), edc((size_t)(pl.ned_alloc / 64), case_id);
   if (int s = stage_copy(pc, err)) return s;
   if (limiter != DGM_LIMITER_OFF) {
-    solver->build_smoothness(S, (int)eo, (int)ne, stream);
+    solver->build_smoothness(S, (int)eo, (int)ne, pstream());
     ACHK(hipGetLastError());
   }
   // the blocks become the case's last, after its data (stream order)
@@ -524,20 +520,29 @@ int Arena::place(const HostMesh& M, const std::vector<double>& patch_q, int case
   return DGM_OK;
 }
 
-int Arena::unplace(const Placement& pl, std::string& err) {
+int Arena::unplace(const Placement& pl, std::string& err, bool release) {
   DevMesh& dm = S.mesh;
+  // -1 in every byte: the blocks belong to no case (kernels already queued still see the case)
   ACHK(hipMemsetAsync(S.lim_pend + pl.e_off, 0xff, pl.ne_alloc * 4, stream));
-  if (int s = set_case64(const_cast<int*>(dm.el_case64), pl.e_off / 64, pl.ne_alloc / 64, -1, err)) return s;
-  if (int s = set_case64(const_cast<int*>(dm.ed_case64), pl.ed_off / 64, pl.ned_alloc / 64, -1, err)) return s;
+  ACHK(hipMemsetAsync(const_cast<int*>(dm.el_case64) + pl.e_off / 64, 0xff, pl.ne_alloc / 64 * 4, stream));
+  ACHK(hipMemsetAsync(const_cast<int*>(dm.ed_case64) + pl.ed_off / 64, 0xff, pl.ned_alloc / 64 * 4, stream));
+  if (release) {
+    ACHK(hipStreamSynchronize(stream));
+    release_ranges(pl);
+  }
+  return DGM_OK;
+}
+
+void Arena::release_ranges(const Placement& pl) {
   r_e.release(pl.e_off / 64, pl.ne_alloc / 64);
   r_ed.release(pl.ed_off / 64, pl.ned_alloc / 64);
   if (pl.npatch > 0) r_patch.release(pl.patch_off, pl.npatch);
-  return DGM_OK;
 }
 
 int Arena::grow_staging(size_t n, std::string& err) {
   if (n <= hbuf_size_) return DGM_OK;
   ACHK(hipStreamSynchronize(stream));
+  if (ustream) ACHK(hipStreamSynchronize(ustream));
   if (hbuf_) {
     ACHK(hipFree(hbuf_));
     bytes -= (int64_t)(hbuf_size_ * sizeof(double));
@@ -559,12 +564,12 @@ int Arena::grow_staging(size_t n, std::string& err) {
 int Arena::upload_state(const double* U, const Placement& pl, std::string& err) {
   const size_t n = (size_t)pl.ne * 4 * T.np;
   if (int s = grow_staging(n, err)) return s;
-  ACHK(hipMemcpyAsync(hbuf_, U, n * 8, hipMemcpyHostToDevice, stream));
-  solver->upload(S, hbuf_, (int)pl.e_off, (int)pl.ne, stream);
+  ACHK(hipMemcpyAsync(hbuf_, U, n * 8, hipMemcpyHostToDevice, pstream()));
+  solver->upload(S, hbuf_, (int)pl.e_off, (int)pl.ne, pstream());
   // nothing of an earlier state stays pending over the uploaded one
-  ACHK(hipMemsetAsync(S.lim_pend + pl.e_off, 0xff, pl.ne_alloc * 4, stream));
+  ACHK(hipMemsetAsync(S.lim_pend + pl.e_off, 0xff, pl.ne_alloc * 4, pstream()));
   ACHK(hipGetLastError());
-  ACHK(hipStreamSynchronize(stream));
+  ACHK(hipStreamSynchronize(pstream()));
   return DGM_OK;
 }
 

@@ -65,8 +65,10 @@ class Arena {
   // marks its 64-blocks as belonging to case_id. Stream-ordered; returns DGM_NO_SPACE if a range does
   // not fit (nothing is changed then).
   int place(const HostMesh& M, const std::vector<double>& patch_q, int case_id, Placement& out, std::string& err);
-  // Marks the case's blocks unused (stream-ordered) and frees its ranges.
-  int unplace(const Placement& pl, std::string& err);
+  // Marks the case's blocks unused, stream-ordered on `stream` without waiting for it, and frees its ranges
+  // (release = false: keeps them allocated; release_ranges then frees them once the stream has passed).
+  int unplace(const Placement& pl, std::string& err, bool release = true);
+  void release_ranges(const Placement& pl);
   // host layout [ne][4][Np] <-> the case's state (stream-ordered; returns when the copy is done)
   int upload_state(const double* U, const Placement& pl, std::string& err);
   int download_state(const double* soa, double* U, const Placement& pl, std::string& err);
@@ -84,12 +86,14 @@ class Arena {
   int64_t bytes = 0;
   RangeAlloc r_e, r_ed, r_patch;  // units of 64 elements, 64 edges, 1 patch
   hipStream_t stream = nullptr;
+  // when set, place() and upload_state() run on this stream instead (a batch admits cases beside its steps)
+  hipStream_t ustream = nullptr;
 
  private:
   template <class X>
   int dalloc(X** p, size_t n, std::string& err);
   int stage_copy(const std::vector<std::pair<void*, std::pair<const void*, size_t>>>& pieces, std::string& err);
-  int set_case64(int* map, int64_t first, int64_t n, int value, std::string& err);
+  hipStream_t pstream() const { return ustream ? ustream : stream; }
   int grow_staging(size_t n, std::string& err);  // device buffer of n doubles for host-layout states
   std::vector<void*> allocs_;
   void* pinned_ = nullptr;

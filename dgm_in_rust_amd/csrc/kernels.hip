@@ -1837,6 +1837,47 @@ void launch_step_begin(Ctl ctl, int ncases, hipStream_t st) {
   if (ncases > 0) k_step_begin<<<ncases, 64, 0, st>>>(ctl, ncases);
 }
 
+namespace {
+__global__ void k_case_poll_init(double* cpar, double* clprev, int* conv, int slot, double mach, double aoa_deg,
+                                 double rho, double p, double hcr) {
+  const double a = sqrt(hcr * p / rho), V = mach * a;
+  const double rad = aoa_deg * (M_PI / 180.0);
+  cpar[4 * slot + 0] = cos(rad);
+  cpar[4 * slot + 1] = sin(rad);
+  cpar[4 * slot + 2] = 1.0 / (0.5 * rho * V * V);
+  cpar[4 * slot + 3] = 0.0;
+  clprev[slot] = 0.0;
+  conv[slot] = 0;
+}
+
+__global__ void k_converge(CaseCtl* cases, const double* __restrict__ force, const double* __restrict__ cpar,
+                           double* clprev, int* conv, double* cl, int n, double chord, double tol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  CaseCtl& c = cases[i];
+  if (!c.live) return;
+  const double v = (force[2 * i + 1] * cpar[4 * i] - force[2 * i] * cpar[4 * i + 1]) * cpar[4 * i + 2] / chord;
+  cl[i] = v;
+  // a case that has not stepped yet (admitted since the last steps) is not tested
+  const bool running = c.active && c.error == ~0ull && (c.step > 0 || c.pending);
+  if (running && !conv[i] && fabs(v - clprev[i]) < tol) {
+    conv[i] = 1;
+    c.final_step = c.step + (c.pending ? 1 : 0);
+  }
+  clprev[i] = v;
+}
+}  // namespace
+
+void launch_case_poll_init(double* cpar, double* clprev, int* conv, int slot, double mach, double aoa_deg, double rho,
+                           double p, double hcr, hipStream_t st) {
+  k_case_poll_init<<<1, 1, 0, st>>>(cpar, clprev, conv, slot, mach, aoa_deg, rho, p, hcr);
+}
+
+void launch_converge(CaseCtl* cases, const double* force, const double* cpar, double* clprev, int* conv, double* cl,
+                     int ncases, double chord, double tol, hipStream_t st) {
+  if (ncases > 0) k_converge<<<(ncases + 255) / 256, 256, 0, st>>>(cases, force, cpar, clprev, conv, cl, ncases, chord, tol);
+}
+
 void launch_set_case(CaseCtl* dst, const CaseCtl& value, hipStream_t st) { k_set_case<<<1, 1, 0, st>>>(dst, value); }
 
 // ---------------------------------------------------------------------------------------

@@ -82,6 +82,15 @@ std::unique_ptr<Solver> make_solver(int precision, int p, int nq, int ne);
 
 // per-case step control: one thread per case slot
 void launch_step_begin(Ctl ctl, int ncases, hipStream_t st);
+// Serving polls. For the case in slot i: its lift normalisation cpar[i] = {cos aoa, sin aoa, 1 / (rho V^2 / 2)},
+// its last lift coefficient (0 at admission) and its convergence mark.
+void launch_case_poll_init(double* cpar, double* clprev, int* conv, int slot, double mach, double aoa_deg, double rho,
+                           double p, double hcr, hipStream_t st);
+// Per slot: the lift coefficient from the wall force (force[2 i], force[2 i + 1]) and chord; a running case
+// whose coefficient moved by less than tol since the previous poll stops after the step it is at
+// (final_step = its step count) and is marked converged.
+void launch_converge(CaseCtl* cases, const double* force, const double* cpar, double* clprev, int* conv, double* cl,
+                     int ncases, double chord, double tol, hipStream_t st);
 // stream-ordered write of one control record by a one-thread kernel
 void launch_set_case(CaseCtl* dst, const CaseCtl& value, hipStream_t st);
 

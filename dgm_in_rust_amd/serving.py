@@ -3,14 +3,16 @@ as capacity frees up, advance together on the GPU and leave when their lift coef
 
 * Admission: a waiting request is added when its arena has a free range for it (its HBM footprint:
   state, RK stage buffers, mesh tables, fluxes, limiter buffers); requests are tried in arrival order
-  and a smaller later one may pass a head that does not fit yet.
+  and a smaller later one may pass a head that does not fit yet. Its mesh and state go to the device
+  beside the queued steps; it takes its first step with the next chunk.
 * Continuous batching: every chunk of ``chunk_steps`` steps advances all live cases in one graph
-  launch per step; requests that arrived meanwhile are admitted between chunks, so a case's steps
-  always start on a chunk boundary.
-* Convergence: at each chunk end a case is finished when |Cl(n) - Cl(n - chunk_steps)| < cl_tol (with
-  chunk_steps = 100: the lift changed by less than the tolerance over the last 100 steps), when it
-  reached ``max_steps``, or when it stopped with an error status.
-* Latency of a request = finish time - arrival time (queueing included).
+  launch per step. The host keeps ``depth`` chunks queued: while the GPU runs one, it reads the poll
+  of the one before, retires finished cases and admits new ones, so the GPU does not wait for it.
+* Convergence, on the device: the poll after each chunk computes every case's lift coefficient and stops
+  a case (after the step it is at) when |Cl(n) - Cl(n - chunk_steps)| < cl_tol; a case also leaves at
+  ``max_steps`` or when it stopped with an error status. A stopped case takes no further step, so the
+  chunk already queued behind its last one leaves it as it was.
+* Latency of a request = time its poll was read - arrival time (queueing included).
 
 The reference has no serving layer; this module drives the dgm_batch_* entry points of libdgm.so.
 """
@@ -55,19 +57,24 @@ class Result:
     def latency(self) -> float:
         return self.finished - self.arrival
 
+    @property
+    def outcome(self) -> str:
+        return "failed" if self.status != 0 else ("converged" if self.converged else "capped")
+
 
 class Server:
     def __init__(self, batch: Batch, chunk_steps: int = 100, cl_tol: float = 1e-5, max_steps: int = 20000,
-                 chord: float = 1.0, keep_states: bool = False):
+                 chord: float = 1.0, keep_states: bool = False, depth: int = 2):
         self.batch = batch
         self.chunk_steps = chunk_steps
         self.cl_tol = cl_tol
         self.max_steps = max_steps
         self.chord = chord
         self.keep_states = keep_states
+        self.depth = max(1, min(2, depth))   # chunks kept queued (the batch holds two poll buffers)
         self.peak_live = 0
-        self.dof_stages = 0.0     # sum over chunks of n_elements * 4 Np * 3 * steps advanced
-        self.busy_seconds = 0.0   # wall time spent in chunks with live cases
+        self.dof_stages = 0.0     # sum over cases of n_elements * 4 Np * 3 * steps taken
+        self.device_ms = 0.0      # device time of the step launches (HIP events), set at the end of run()
 
     def run(self, requests: List[Request], time_limit: Optional[float] = None,
             clock: Optional[Callable[[], float]] = None, idle: Optional[Callable[[float], None]] = None) -> List[Result]:
@@ -80,53 +87,73 @@ class Server:
             clock = lambda: time.perf_counter() - t0  # noqa: E731
         if idle is None:
             idle = time.sleep
+        dev0 = b.device_ms()
         todo = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting: List[Request] = []
-        live: Dict[int, dict] = {}     # case id -> request, admission time, last Cl
+        live: Dict[int, dict] = {}     # case id -> request, admission time, first poll, steps counted
+        queued = collections.deque()  # tickets of the polls queued behind each chunk
+        polls_queued = 0
         results: List[Result] = []
-        while todo or waiting or live:
-            now = clock()
-            if time_limit is not None and now > time_limit:
-                break
-            while todo and todo[0].arrival <= now:
-                waiting.append(todo.popleft())
-            still = []
+
+        def admit() -> int:
+            nonlocal waiting
+            still, n = [], 0
             for r in waiting:
                 cid = b.add(r.mesh, r.order, r.mach, r.aoa_deg, final_step=self.max_steps)
                 if cid is None:
                     still.append(r)
                     continue
-                live[cid] = dict(req=r, admitted=clock(), cl=0.0, ne=int(sum(int(c) for c in r.mesh.el_count)),
-                                 steps=0)
+                live[cid] = dict(req=r, admitted=clock(), first_poll=polls_queued,
+                                 ne=int(sum(int(c) for c in r.mesh.el_count)), steps=0)
+                n += 1
             waiting = still
+            return n
+
+        while todo or waiting or live or queued:
+            now = clock()
+            if time_limit is not None and now > time_limit:
+                break
+            while todo and todo[0].arrival <= now:
+                waiting.append(todo.popleft())
+            admit()
             self.peak_live = max(self.peak_live, len(live))
-            if not live:
+            # keep the GPU fed: queue chunks (each followed by its poll) up to the depth
+            if live and len(queued) < self.depth:
+                b.steps_async(self.chunk_steps)
+                queued.append(b.poll_async(self.chord, self.cl_tol))
+                polls_queued += 1
+                continue
+            if not queued:   # nothing live
+                if waiting:
+                    # the slots and ranges of removed cases come back once the device has passed the removal
+                    b.synchronize()
+                    if not admit():
+                        raise RuntimeError(f"request {waiting[0].rid} does not fit in the empty batch")
+                    continue
                 if todo:
                     idle(max(0.0, todo[0].arrival - clock()))
                 continue
-            c0 = clock()
-            b.steps_async(self.chunk_steps)
-            b.synchronize()
-            info = b.info()
-            cl = b.lift(self.chord)
+            ticket = queued.popleft()
+            info, cl, conv = b.poll_wait(ticket, max(live) + 1 if live else 1)
             now = clock()
-            self.busy_seconds += now - c0
             for cid in list(live):
                 rec = live[cid]
+                if rec["first_poll"] > ticket:   # admitted after this poll was queued
+                    continue
                 inf = info[cid]
                 advanced = inf["steps"] - rec["steps"]
                 rec["steps"] = inf["steps"]
                 self.dof_stages += rec["ne"] * 4 * n_basis(rec["req"].order) * 3 * advanced
-                dcl = abs(cl[cid] - rec["cl"])
-                rec["cl"] = cl[cid]
-                converged = inf["status"] == 0 and dcl < self.cl_tol
-                if converged or not inf["active"]:
+                if conv[cid] or not inf["active"]:
                     r = rec["req"]
                     state = b.state(cid) if self.keep_states else None
                     results.append(Result(r.rid, r.order, rec["ne"], r.arrival, rec["admitted"], now, inf["steps"],
-                                          bool(converged), int(inf["status"]), float(cl[cid]), state))
+                                          bool(conv[cid]) and inf["status"] == 0, int(inf["status"]), float(cl[cid]),
+                                          state))
                     b.remove(cid)
                     del live[cid]
+        b.synchronize()
+        self.device_ms = b.device_ms() - dev0
         return results
 
 
@@ -136,17 +163,34 @@ def poisson_arrivals(rate: float, n: int, seed: int = 0) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate, size=n))
 
 
+def _pct(x, q):
+    return float(np.percentile(x, q)) if len(x) else None
+
+
 def summary(results: List[Result], server: Server, wall: float) -> dict:
+    """Latency percentiles over all requests and per outcome (converged, capped at max_steps, failed),
+    throughput, aggregate GDOF * stage / s over the device time of the steps, and the GPU-busy fraction
+    (device time of the step launches / wall time)."""
     lat = np.array([r.latency for r in results]) if results else np.zeros(0)
+    by = {}
+    for outcome in ("converged", "capped", "failed"):
+        x = np.array([r.latency for r in results if r.outcome == outcome])
+        st = [r.steps for r in results if r.outcome == outcome]
+        by[outcome] = dict(count=int(len(x)), p50_latency_s=_pct(x, 50), p99_latency_s=_pct(x, 99),
+                           mean_steps=float(np.mean(st)) if st else None)
+    dev_s = server.device_ms / 1e3
     return dict(
         completed=len(results),
-        converged=int(sum(r.converged for r in results)),
-        failed=int(sum(r.status != 0 for r in results)),
-        p50_latency_s=float(np.percentile(lat, 50)) if len(lat) else None,
-        p99_latency_s=float(np.percentile(lat, 99)) if len(lat) else None,
+        converged=by["converged"]["count"],
+        capped=by["capped"]["count"],
+        failed=by["failed"]["count"],
+        p50_latency_s=_pct(lat, 50),
+        p99_latency_s=_pct(lat, 99),
+        by_outcome=by,
         mean_steps=float(np.mean([r.steps for r in results])) if results else None,
         throughput_cases_per_s=len(results) / wall if wall > 0 else None,
-        gdof_stage_per_s=server.dof_stages / server.busy_seconds / 1e9 if server.busy_seconds > 0 else None,
+        gdof_stage_per_s=server.dof_stages / dev_s / 1e9 if dev_s > 0 else None,
+        gpu_busy_fraction=dev_s / wall if wall > 0 else None,
         peak_concurrent=server.peak_live,
         wall_s=wall,
     )

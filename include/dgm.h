@@ -242,7 +242,11 @@ int dgm_batch_last_error(const dgm_batch* b, dgm_error* err);
 int dgm_batch_footprint(const dgm_batch* b, int32_t order, int64_t n_elements, int64_t n_edges, int64_t n_patches,
                         int64_t* bytes);
 /* Admits a case: initial state U0 in the reference layout [ne][4][Np] (NULL: the uniform far-field state,
- * Solver::set_initial_solution src/solver.rs:45-79). DGM_NO_SPACE when its arena has no free range now. */
+ * Solver::set_initial_solution src/solver.rs:45-79). DGM_NO_SPACE when its arena has no free range now.
+ * Its mesh, state and first time step are written beside the steps already queued (which it does not
+ * join); it takes its first step with the next dgm_batch_steps_async. dgm_batch_remove is stream-ordered
+ * too and does not wait: a removed case's slot and ranges return to use once the steps queued before the
+ * removal have run (after a later poll or a synchronisation). */
 int dgm_batch_add(dgm_batch* b, const dgm_mesh* mesh, const dgm_case_params* params, const double* U0,
                   int32_t* case_id);
 int dgm_batch_remove(dgm_batch* b, int32_t case_id);
@@ -254,6 +258,17 @@ int dgm_batch_case_info(dgm_batch* b, int32_t n, dgm_case_info* out);
 int dgm_batch_case_state(dgm_batch* b, int32_t case_id, double* U);
 /* lift coefficient of every slot 0 .. n-1 (NaN for empty slots), chord c */
 int dgm_batch_lift(dgm_batch* b, double chord, int32_t n, double* cl);
+/* Serving support (extension). Enqueues, after the work already queued: every live case's lift coefficient
+ * (chord c); for every running case whose coefficient moved by less than cl_tol since the previous poll (0 at
+ * admission), a stop after the step it is at, marked converged; and a copy of every slot's record, coefficient
+ * and mark into host memory of the batch. Returns at once with a ticket; at most two polls may be outstanding. */
+int dgm_batch_poll_async(dgm_batch* b, double chord, double cl_tol, int64_t* ticket);
+/* Waits for the oldest outstanding poll (ticket) and reports slots 0 .. n-1 as they were when it ran (any
+ * output may be NULL). Cases admitted and removed meanwhile are not reflected; cases admitted while steps
+ * run go live between steps, so a poll queued before a case's first step shows it not live. */
+int dgm_batch_poll_wait(dgm_batch* b, int64_t ticket, int32_t n, dgm_case_info* info, double* cl, int32_t* converged);
+/* device time (ms) of the step launches so far (events around each dgm_batch_steps_async call; completed ones) */
+int dgm_batch_device_time(dgm_batch* b, double* ms);
 int dgm_batch_get_tuning(const dgm_batch* b, int32_t order, dgm_tuning* t);
 int dgm_batch_set_tuning(dgm_batch* b, int32_t order, const dgm_tuning* t);
 /* Times the launch shapes of every served order on the resident cases (stage-0 passes only, which

@@ -65,6 +65,7 @@ def test_continuous_batching_add_remove_midrun():
     assert b.info(8)[a]["steps"] == 20 and b.info(8)[bb]["steps"] == 10
     UA_mid = b.state(a)
     b.remove(a)
+    b.synchronize()   # the removed case's ranges come back once the device has passed the removal
     c = b.add(mC, 2, 0.5, -1.0, final_step=15)
     assert c is not None
     b.steps_async(25)
@@ -91,6 +92,10 @@ def test_admission_by_capacity():
     assert hc.info.n_elements == 1600
     assert b.info(4)[ids[0]]["footprint_bytes"] == fp > 1600 * 100
     b.remove(ids[0])
+    # the removed case's slot and ranges stay out of use until the device has passed the removal
+    # (the next completed poll, or a synchronisation)
+    assert b.add(m, 2, 0.5, 0.0) is None
+    b.synchronize()
     assert b.add(m, 2, 0.5, 0.0) is not None
     with pytest.raises(Exception):
         b.add(m, 3, 0.5, 0.0)   # no arena for p = 3

@@ -144,3 +144,33 @@ def test_sweep_case_positivity_failure_matches_oracle(precision):
         assert abs(got["steps"] - (no + 1)) <= 0.02 * no
     for o in others:   # (the oracle runs both 200 steps without a failure)
         assert info[o]["status"] == 0 and info[o]["steps"] == 200
+
+
+def test_served_failure_is_reported_as_failed():
+    """The failing sweep case served beside two healthy ones (chunks of 50 steps, a step cap of 700, a
+    tolerance no lift change meets): it leaves with the oracle's status after the oracle's step count,
+    the others leave at the cap, and the summary splits them by outcome."""
+    from dgm_in_rust_amd.serving import summary
+    c, mesh = failing_sweep_case()
+    so, no, bad = failing_case_oracle()
+    healthy = [small_sweep()[i] for i in (0, 2)]
+    pairs = [(c, mesh)] + healthy
+    reqs = [Request(i, 0.0, m, cc["order"], cc["mach"], cc["aoa_deg"]) for i, (cc, m) in enumerate(pairs)]
+    cap = {p: 0 for p in (1, 2, 3)}
+    for cc, m in pairs:
+        cap[cc["order"]] += (m.n_elements + 63) // 64 * 64
+    clk = SimClock()
+    with Batch(cap, precision="fp64", limiter="jacobi", max_cases=4) as b:
+        srv = Server(b, chunk_steps=50, cl_tol=0.0, max_steps=700)
+        res = srv.run(reqs, clock=clk, idle=clk.idle)
+    s = summary(res, srv, clk.t)
+    by = {r.rid: r for r in res}
+    assert len(res) == 3
+    f = by[0]
+    print(f"served failing case: status {f.status} after {f.steps} steps; oracle: {so} after {no}", flush=True)
+    assert f.status == so and f.steps == no + 1 and f.outcome == "failed" and not f.converged
+    for rid in (1, 2):
+        r = by[rid]
+        assert r.steps == 700 and r.status == 0 and r.outcome == "capped", (rid, r.status, r.steps)
+    assert s["failed"] == 1 and s["capped"] == 2 and s["converged"] == 0
+    assert s["by_outcome"]["failed"]["count"] == 1 and s["by_outcome"]["converged"]["p50_latency_s"] is None

@@ -57,3 +57,32 @@ def test_served_results_equal_alone():
         U, cl = alone(meshgen.naca0012_ogrid(*m[:2], seed=m[2]), p, M, a, r.steps)
         assert np.array_equal(r.state, U), r.rid
         assert abs(r.cl - cl) <= 1e-12 * max(1.0, abs(cl)), (r.rid, r.cl, cl)
+
+
+def test_poll_stops_converged_cases():
+    """A poll with a tolerance no lift change meets stops nothing; one that every change meets stops each
+    running case after the step it is at, so later steps leave it as it was. The poll's lift equals the
+    batch lift read-out, and a case admitted after the steps were queued is not tested."""
+    m1 = meshgen.naca0012_ogrid(40, 20, seed=61)
+    m2 = meshgen.naca0012_ogrid(32, 12, seed=62)
+    with Batch({2: 4096}, precision="fp32", limiter="jacobi", max_cases=4) as b:
+        a = b.add(m1, 2, 0.6, 2.0, final_step=1000)
+        b.steps_async(20)
+        t0 = b.poll_async(1.0, 0.0)
+        late = b.add(m2, 2, 0.5, 1.0, final_step=1000)   # goes live with the next steps
+        t1 = b.poll_async(1.0, 1.0e9)                     # queued before any step of `late`
+        info0, cl0, conv0 = b.poll_wait(t0)
+        info1, cl1, conv1 = b.poll_wait(t1)
+        assert info0[a]["steps"] == 20 and not conv0[a] and info0[late]["live"] == 0
+        assert conv1[a] and not conv1[late] and info1[late]["steps"] == 0
+        b.steps_async(20)
+        b.synchronize()
+        info = b.info(4)
+        assert info[a]["steps"] == 20 and info[a]["active"] == 0 and info[a]["status"] == 0
+        assert info[late]["steps"] == 20 and info[late]["active"] == 1
+        lift = b.lift(1.0, 4)
+        Ua = b.state(a)
+    U, cl = alone(m1, 2, 0.6, 2.0, 20)
+    assert np.array_equal(Ua, U)
+    assert abs(cl0[a] - cl) <= 1e-12 * max(1.0, abs(cl)) and abs(lift[a] - cl) <= 1e-12 * max(1.0, abs(cl))
+    assert abs(cl1[a] - cl0[a]) <= 1e-12 * max(1.0, abs(cl))   # same state; atomics may reorder the force sum
