@@ -89,7 +89,39 @@ def summarise(out, title, args, copy_mib=256):
         gbs = t * 1e6 / (r["avg"] * 1e-6) / 1e9 if r["avg"] > 0 else float("nan")
         lines.append("%-40s %6d %9.2f %9.2f %9.2f %9.3f %11.1f %11.1f %10.2f %8.0f %8.3f" %
                      (k, r["calls"], r["avg"], r["mn"], r["mx"], r["tot"], f, w, t, gbs, gbs / HBM_PEAK_GBS))
+    sq = os.path.join(out, "sq", "pmc_counter_collection.csv")
+    if os.path.exists(sq):
+        lines += instruction_table(sq, args)
     return "\n".join(lines) + "\n"
+
+
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VALU_MFMA_F32", "SQ_INSTS_SALU")
+
+
+def instruction_table(path, args):
+    """Per-kernel instruction counts of an SQ counter run (rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU
+    SQ_INSTS_VALU_MFMA_F32 SQ_INSTS_SALU): wavefronts per dispatch and instructions per wavefront.
+    The rows start with "sq" so that read_traffic() skips them."""
+    cols = {c: read_counter(path, c)[0] for c in SQ_COUNTERS}
+    keys = sorted(set().union(*[set(v) for v in cols.values()]))
+    lines = ["#",
+             "# instruction counts from a fourth run of the same command:",
+             f"#   rocprofv3 --pmc {' '.join(SQ_COUNTERS)} --output-format csv -d OUT/sq -o pmc -- python3 R/bench.py {args}",
+             "# waves: wavefronts per dispatch; valu, mfma, salu: instructions per wavefront (dispatch means)",
+             "%-43s %10s %9s %8s %8s" % ("sq kernel", "waves", "valu", "mfma", "salu")]
+
+    def mean(c, k):
+        v = cols[c].get(k)
+        return sum(v) / len(v) if v else float("nan")
+
+    for k in keys:
+        w = mean("SQ_WAVES", k)
+        if not (w > 0):
+            continue
+        lines.append("sq %-40s %10.0f %9.1f %8.1f %8.1f" % (k, w, mean("SQ_INSTS_VALU", k) / w,
+                                                          mean("SQ_INSTS_VALU_MFMA_F32", k) / w,
+                                                          mean("SQ_INSTS_SALU", k) / w))
+    return lines
 
 
 def trace_summary(out, title, command):

@@ -64,3 +64,29 @@ def test_trace_only_summary(tmp_path):
     assert lines[1].split()[0] == "k_limit_walk<double,2,6,3,0>"
     assert float(lines[1].split()[2]) == pytest.approx(880.0)
     assert "--ogrid --steps 100" in text
+
+
+def test_instruction_table_per_wave(tmp_path):
+    """The SQ run's counters become wavefronts per dispatch and instructions per wavefront, in rows that
+    read_traffic() does not take for kernel rows."""
+    out = str(tmp_path)
+    stats(out, [[SURF, 2, 30000, 15000, 50, 14000, 16000, 1], [COPY, 1, 100000, 100000, 50, 100000, 100000, 1]])
+    kib = 256 * 1024
+    pmc(out, "fetch", "FETCH_SIZE", [[COPY, 1 << 26, kib / 2], [SURF, 4096, 1000.0]])
+    pmc(out, "write", "WRITE_SIZE", [[COPY, 1 << 26, kib], [SURF, 4096, 500.0]])
+    rows = []
+    for waves, valu, mfma, salu in ((3000, 3.6e6, 0, 1.2e6), (3000, 3.9e6, 0, 1.5e6)):
+        rows += [[SURF, 4096, "SQ_WAVES", waves], [SURF, 4096, "SQ_INSTS_VALU", valu],
+                 [SURF, 4096, "SQ_INSTS_VALU_MFMA_F32", mfma], [SURF, 4096, "SQ_INSTS_SALU", salu]]
+    write_csv(os.path.join(out, "sq", "pmc_counter_collection.csv"),
+              ["Kernel_Name", "Grid_Size", "Counter_Name", "Counter_Value"], rows)
+    text = rs.summarise(out, "t", "--steps 1")
+    row = [l for l in text.splitlines() if l.startswith("sq k_surface")]
+    assert len(row) == 1
+    f = row[0].split()
+    assert float(f[2]) == 3000 and float(f[3]) == pytest.approx(1250.0) and float(f[4]) == 0.0
+    assert float(f[5]) == pytest.approx(450.0)
+    path = tmp_path / "s.txt"
+    path.write_text(text)
+    traffic, us = rs.read_traffic(str(path), "k_surface<float")
+    assert traffic == pytest.approx((1000.0 * 2 + 500.0) * 1024.0, rel=1e-6) and us == pytest.approx(15.0)
