@@ -49,7 +49,7 @@ class Info(C.Structure):
                 ("n_internal_elements", C.c_int64), ("n_boundary_elements", C.c_int64), ("n_basis", C.c_int32),
                 ("n_cell_gp", C.c_int32), ("n_edge_gp", C.c_int32), ("order", C.c_int32),
                 ("precision", C.c_int32), ("limiter", C.c_int32), ("n_limiter_levels", C.c_int32),
-                ("cfl", C.c_double), ("device_bytes", C.c_int64)]
+                ("fused_pass", C.c_int32), ("cfl", C.c_double), ("device_bytes", C.c_int64)]
 
 
 class Error(C.Structure):

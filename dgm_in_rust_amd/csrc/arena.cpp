@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 
@@ -89,6 +90,7 @@ CaseCtl fresh_case(double cfl) {
   h.cfl = cfl;
   h.error = ~0ull;
   h.error_s0 = ~0ull;
+  for (int s = 0; s < 3; ++s) h.error_f[s] = ~0ull;
   h.final_step = 0x7fffffff;
   h.active = 1;
   h.live = 1;
@@ -157,12 +159,13 @@ Footprint Arena::footprint(int order, int nq, int ne_pts, int precision, int lim
   Footprint f;
   int64_t e = 0;
   e += nc * 8;                            // u^n
-  e += f32 ? 3 * nc * 4 : nc * 8;         // shadows + increment, or the FP64 stage state
+  e += f32 ? 4 * nc * 4 : nc * 8;         // shadows + increment, or the FP64 stage state
   if (residual_buffer) e += nc * 8;       // residual output
   e += 3 * 4 + 3 * 4;                     // edge ids, neighbours
   e += G_N * 8 + G_N * 4;                 // geometry (FP64 + FP32 copy)
   e += 4 + nc * 8 + 1;                    // troubled list, rebuilt polynomials, flags
   if (limiter != DGM_LIMITER_OFF) e += np * (np + 1) / 2 * 8;  // smoothness matrix
+  if (f32 && order == 3) e += 3 * 4 + GRP_EDGES * 32 / 16 + 1;  // fused pass: face slots, group edge records
   e += 1;                                 // case map (4 bytes per 64 elements, rounded up)
   f.per_element = e;
   int64_t d = 0;
@@ -348,15 +351,18 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
   if (int s = dalloc(&S.un, ns, err)) return s;
   ACHK(hipMemset(S.un, 0, ns * 8));
   if (f32) {
-    float *a, *b, *d;
+    float *a, *b, *b2, *d;
     if (int s = dalloc(&a, ns, err)) return s;
     if (int s = dalloc(&b, ns, err)) return s;
+    if (int s = dalloc(&b2, ns, err)) return s;
     if (int s = dalloc(&d, ns, err)) return s;
     ACHK(hipMemset(a, 0, ns * 4));
     ACHK(hipMemset(b, 0, ns * 4));
+    ACHK(hipMemset(b2, 0, ns * 4));
     ACHK(hipMemset(d, 0, ns * 4));
     S.sn = a;
     S.s = b;
+    S.s2 = b2;
     S.d = d;
   } else {
     double* a;
@@ -391,6 +397,32 @@ int Arena::init(int order_, int nq, int ne_pts, int precision_, int limiter_, in
   // than the one-lane-per-element kernel (measured on the 128k-triangle O-grid: p = 1 18.2 vs 9.0 us,
   // p = 2 19.5 vs 13.2 us, p = 3 23.3 vs 27.2 us per stage)
   if (order <= 2) S.tune.element_waves = 0;
+  // The fused edge-and-element pass of a march (FP32-bulk at p = 3, on the matrix cores) and its group
+  // edge records, with DGM_FUSED=1 in the environment at set-up. It gives the separate passes' results
+  // bit for bit but is not the default: on the 128k-triangle O-grid it takes 43 us per stage against
+  // 14.4 + 23.0 us for the edge and element kernels (a 16-element group evaluates 2.06 edges per element
+  // against 1.5, and spends 1.6 times the vector instructions of the two kernels together).
+  {
+    const char* fz = std::getenv("DGM_FUSED");
+    if (f32 && S.use_mfma && order == 3 && fz && fz[0] == '1') {
+      int4* gr;
+      float4* gg;
+      int *gn, *gs;
+      if (int s = dalloc(&gr, (size_t)cap_e / 16 * GRP_EDGES, err)) return s;
+      if (int s = dalloc(&gg, (size_t)cap_e / 16 * GRP_EDGES, err)) return s;
+      if (int s = dalloc(&gn, (size_t)cap_e / 16, err)) return s;
+      if (int s = dalloc(&gs, (size_t)3 * cap_e, err)) return s;
+      ACHK(hipMemset(gr, 0, (size_t)cap_e / 16 * GRP_EDGES * sizeof(int4)));
+      ACHK(hipMemset(gg, 0, (size_t)cap_e / 16 * GRP_EDGES * sizeof(float4)));
+      ACHK(hipMemset(gn, 0, (size_t)cap_e / 16 * 4));
+      ACHK(hipMemset(gs, 0, (size_t)3 * cap_e * 4));
+      dm.grp_rec = gr;
+      dm.grp_geo = gg;
+      dm.grp_n = gn;
+      dm.el_gslot = gs;
+      S.fused = 1;
+    }
+  }
   // rebuild wavefronts: enough for a troubled fraction of 1 / 32 in one pass, 2048 at least
   S.tune.weno_grid = (int)std::min<int64_t>(65536, std::max<int64_t>(2048, cap_e / 32));
   dm.hcr = 1.4;
@@ -505,6 +537,43 @@ int Arena::place(const HostMesh& M, const std::vector<double>& patch_q, int case
   }
   if (pl.npatch > 0)
     pc.push_back({(void*)(dm.patch_q + 4 * pl.patch_off), {patch_q.data(), (size_t)pl.npatch * 4 * 8}});
+  // group edge lists of the fused pass: per 16 elements (the groups of the 64-aligned range), the
+  // distinct edges of the case's elements in increasing order, and each face's place in its list
+  const int64_t ngrp = pl.ne_alloc / 16;
+  std::vector<int4> grp_rec;
+  std::vector<float4> grp_geo;
+  std::vector<int> grp_n, el_gslot;
+  if (dm.grp_rec) {
+    grp_rec.assign((size_t)ngrp * GRP_EDGES, int4{0, -1, 0, 0});
+    grp_geo.assign((size_t)ngrp * GRP_EDGES, float4{0.f, 0.f, 0.f, 0.f});
+    grp_n.assign((size_t)ngrp, 0);
+    el_gslot.assign((size_t)3 * ne, 0);
+    for (int64_t gi = 0; gi < ngrp; ++gi) {
+      const int64_t lo = gi * 16, hi = std::min<int64_t>(ne, lo + 16);
+      std::vector<int64_t> ids;
+      for (int64_t e = lo; e < hi; ++e)
+        for (int i = 0; i < 3; ++i) ids.push_back(M.el_edge[3 * e + i]);
+      std::sort(ids.begin(), ids.end());
+      ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+      grp_n[gi] = (int)ids.size();
+      for (size_t k = 0; k < ids.size(); ++k) {
+        const int64_t id = ids[k];
+        const bool interior = ((ed_info[id] >> 4) & 3) == BC_INTERIOR;
+        grp_rec[(size_t)gi * GRP_EDGES + k] = int4{ed_elem[id], interior ? ed_elem[ned + id] : -1, ed_info[id], (int)(edo + id)};
+        grp_geo[(size_t)gi * GRP_EDGES + k] = float4{ed_geo32[id], ed_geo32[ned + id], ed_geo32[2 * ned + id], 0.f};
+      }
+      for (int64_t e = lo; e < hi; ++e)
+        for (int i = 0; i < 3; ++i) {
+          const int k = (int)(std::lower_bound(ids.begin(), ids.end(), M.el_edge[3 * e + i]) - ids.begin());
+          el_gslot[i * ne + e] = (k << 1) | (el_edge[i * ne + e] & 1);
+        }
+    }
+    for (int i = 0; i < 3; ++i)
+      pc.push_back({(void*)(const_cast<int*>(dm.el_gslot) + i * nes + eo), {el_gslot.data() + i * ne, ne * 4}});
+    pc.push_back({(void*)(const_cast<int4*>(dm.grp_rec) + eo / 16 * GRP_EDGES), {grp_rec.data(), grp_rec.size() * sizeof(int4)}});
+    pc.push_back({(void*)(const_cast<float4*>(dm.grp_geo) + eo / 16 * GRP_EDGES), {grp_geo.data(), grp_geo.size() * sizeof(float4)}});
+    pc.push_back({(void*)(const_cast<int*>(dm.grp_n) + eo / 16), {grp_n.data(), grp_n.size() * 4}});
+  }
   std::vector<int> elc((size_t)(pl.ne_alloc / 64), case_id), edc((size_t)(pl.ned_alloc / 64), case_id);
   if (int s = stage_copy(pc, err)) return s;
   if (limiter != DGM_LIMITER_OFF) {
@@ -675,7 +744,8 @@ int autotune_steps(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned l
       A->S.tune = best;
       return DGM_OK;
     };
-    status = pick([](Tuning& t, int v) { t.surface_block = v; }, blocks, 3);
+    // (with the fused edge-and-element pass no edge kernel runs in a march)
+    if (!A->S.fused) status = pick([](Tuning& t, int v) { t.surface_block = v; }, blocks, 3);
     if (!status) {
       // the shape of the element kernel variant in use. The variant itself (matrix-core, or 0: one lane
       // per element) is not searched: the two round FP32 sums differently, and a case must give the

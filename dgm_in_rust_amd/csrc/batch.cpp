@@ -245,9 +245,10 @@ int ensure_graph(dgm_batch* b) {
 
 int case_status_code(const CaseCtl& h, int64_t* edge) {
   *edge = -1;
-  if (h.error == ~0ull) return DGM_OK;
-  const int code = (int)(h.error & 0xff);
-  const int64_t e = (int64_t)(h.error >> 8);
+  const unsigned long long err = case_error(h);
+  if (err == ~0ull) return DGM_OK;
+  const int code = (int)(err & 0xff);
+  const int64_t e = (int64_t)(err >> 8);
   *edge = e == 0xFFFFFF ? -1 : e;
   if (code == 1) return DGM_NEG_DENSITY;
   if (code == 2) return DGM_NEG_PRESSURE;
@@ -302,6 +303,7 @@ int dgm_batch_create(const dgm_batch_params* params, dgm_batch** out) {
   std::memset(b->h_act, 0, sizeof(CaseCtl) * nc);
   std::memset(b->h_empty, 0, sizeof(CaseCtl));
   b->h_empty->error = b->h_empty->error_s0 = ~0ull;
+  for (int s = 0; s < 3; ++s) b->h_empty->error_f[s] = ~0ull;
   if ((s = balloc(b.get(), &b->d_cpar, 4 * (size_t)nc))) return cleanup(s);
   if ((s = balloc(b.get(), &b->d_clprev, nc))) return cleanup(s);
   if ((s = balloc(b.get(), &b->d_conv, nc))) return cleanup(s);

@@ -136,9 +136,10 @@ CaseCtl fresh_ctl(const dgm_ctx* c) {
 
 // status word of the device: lowest failing edge and its code
 int ctl_status(dgm_ctx* c, const CaseCtl& h) {
-  if (h.error == ~0ull) return DGM_OK;
-  const int code = (int)(h.error & 0xff);
-  const int64_t edge = (int64_t)(h.error >> 8);
+  const unsigned long long err = case_error(h);
+  if (err == ~0ull) return DGM_OK;
+  const int code = (int)(err & 0xff);
+  const int64_t edge = (int64_t)(err >> 8);
   if (code == 1) return fail(c, DGM_NEG_DENSITY, "Negative density found in HLLC!", edge);
   if (code == 2) return fail(c, DGM_NEG_PRESSURE, "Negative pressure found in HLLC!", edge);
   if (code == 5) return fail(c, DGM_BAD_MESH, "Invalid number of boundary neighbours!", -1, edge == 0xFFFFFF ? -1 : edge);
@@ -322,6 +323,7 @@ int dgm_ctx_info(const dgm_ctx* c, dgm_info* info) {
   info->precision = c->prm.precision;
   info->limiter = c->prm.limiter;
   info->n_limiter_levels = (int32_t)c->M.level_offsets.size() - 1;
+  info->fused_pass = (c->A && c->A->S.fused && c->A->S.use_mfma && c->A->S.tune.element_waves > 0) ? 1 : 0;
   info->cfl = c->cfl;
   info->device_bytes = c->A ? c->A->bytes : 0;
   return DGM_OK;
@@ -566,7 +568,7 @@ int dgm_device_time_march(dgm_ctx* c, int32_t final_step, double final_time, dgm
     for (int i = 0; i < batch; ++i) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
     if (int s = read_ctl(c, h)) return s;
     effective_clock(h, t, steps);
-    if (h.error != ~0ull || !h.active || steps >= final_step || !(t < final_time)) break;
+    if (case_error(h) != ~0ull || !h.active || steps >= final_step || !(t < final_time)) break;
   }
   if (stats) {
     stats->steps = steps;

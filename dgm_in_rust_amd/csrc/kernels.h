@@ -42,7 +42,17 @@ struct DevMesh {
   const int* ed_case64;    // [edge capacity / 64] case of each 64-edge block, -1 unused
   int ed_stride;           // edge capacity (stride of ed_elem / ed_geo)
   int single_case;         // >= 0: every block belongs to this case (no map lookup); -1: use the maps
+  // The fused edge-and-element pass (FP32-bulk, p = 3, DGM_FUSED=1): per group of 16 elements (one wavefront), the
+  // records of the distinct edges of its elements in increasing edge order, j < grp_n[g]: grp_rec =
+  // {left element, right element or -1 (boundary), info word, edge id}, grp_geo = {nx, ny, half length, 0}
+  // (the edge arrays' values, gathered per group at set-up so that a wavefront reads them in one trip);
+  // el_gslot[i][e] = (place of face i's edge in its group's list) << 1 | side. nullptr without that pass.
+  const int4* grp_rec;     // [nes / 16][GRP_EDGES]
+  const float4* grp_geo;   // [nes / 16][GRP_EDGES]
+  const int* grp_n;        // [nes / 16]
+  const int* el_gslot;     // [3][nes]
 };
+constexpr int GRP_EDGES = 48;  // at most three distinct edges per element of a 16-element group
 
 // Step control of one case, resident on the device so that whole steps replay from a captured graph.
 // A step is counted at the start of the next one (pending): the time-step reduction for step n + 1
@@ -62,9 +72,20 @@ struct CaseCtl {
   // runs beside the step control, which reads error, so it records here and the stage-0 element pass
   // moves it into error when the case runs the step
   unsigned long long error_s0;
-  int pad[4];
+  // First failing edge of the fused edge-and-element pass of stage s (same coding as error). Every
+  // wavefront of such a pass decides at its start whether the case runs, so the pass records into a
+  // word that none of its own wavefronts reads; the passes after it stop the case on it (case_error).
+  unsigned long long error_f[3];
+  int pad[2];
 };
-static_assert(sizeof(CaseCtl) == 128, "CaseCtl is one 128-byte record");
+static_assert(sizeof(CaseCtl) == 144, "CaseCtl is one 144-byte record");
+
+// the error status of a case: its error word and those of the fused passes
+__host__ __device__ inline unsigned long long case_error(const CaseCtl& c) {
+  unsigned long long e = c.error;
+  for (int s = 0; s < 3; ++s) e = c.error_f[s] < e ? c.error_f[s] : e;
+  return e;
+}
 
 // The time-step minimum of a case is reduced into DT_LANES words 128 bytes apart: atomics from many
 // workgroups to one word serialise in L2, so each wavefront picks a word by its index.
@@ -88,7 +109,10 @@ struct StageBufs {
   double* un;      // u^n (FP64)
   T* us;           // FP64 mode: stage state
   T* sn;           // FP32-bulk: float shadow of u^n
-  T* s;            // FP32-bulk: float stage state
+  T* s;            // FP32-bulk: float state of stage 1, u^(1) (written by stage 0)
+  T* s2;           // FP32-bulk: float state of stage 2, u^(2) (written by stage 1). A stage never writes
+                   // the array it reads, so kernels that read the stage input of other elements may run
+                   // beside the update of their own.
   T* d;            // FP32-bulk: stage increment u^(s) - u^n
   double* rout;    // OP_RESID output, [4 Np][nes]
   // Deferred limiter output (Jacobi march): pend[e] >= 0 means element e's stage input is the rebuilt

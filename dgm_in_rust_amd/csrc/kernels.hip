@@ -253,7 +253,7 @@ __device__ __forceinline__ void record_error(CaseCtl* cc, int edge, int code) {
   atomicMin(&cc->error, ((unsigned long long)(unsigned)edge << 8) | (unsigned long long)code);
 }
 
-__device__ __forceinline__ bool case_runs(const CaseCtl& cc) { return cc.active && cc.error == ~0ull; }
+__device__ __forceinline__ bool case_runs(const CaseCtl& cc) { return cc.active && case_error(cc) == ~0ull; }
 
 // Case of a 64-block, read through the scalar unit: the block index is uniform over the wavefront
 // (every caller's wavefront covers one 64-block), so the case record that follows is a scalar load.
@@ -277,7 +277,7 @@ __device__ __forceinline__ void store_limited(const StageBufs<T>& b, int nes, in
       b.un[ix] = val;
       b.sn[ix] = (T)val;
     } else {
-      b.s[ix] = (T)val;
+      (STAGE == 0 ? b.s : b.s2)[ix] = (T)val;
       b.d[ix] = (T)(val - b.un[ix]);
     }
   }
@@ -298,7 +298,7 @@ __device__ __forceinline__ void store_limited_row(const StageBufs<T>& b, int nes
   for (int c = 0; c < 4 * NP; ++c) {
     const size_t ix = (size_t)c * nes + e;
     if constexpr (DN) {
-      b.s[ix] = (T)v[c];
+      (STAGE == 0 ? b.s : b.s2)[ix] = (T)v[c];
       b.d[ix] = (T)(v[c] - un[c]);
     } else {
       store_limited<T, NP, STAGE>(b, nes, e, c, v[c]);
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
   CaseCtl* cc = ctl.cases + (cs < 0 ? 0 : cs);
   const int c_active = s0 ? __hip_atomic_load(&cc->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : cc->active;
   const int c_ed_end = cc->ed_end;
-  const unsigned long long c_error = cc->error;
+  const unsigned long long c_error = case_error(*cc);
   for (int i = threadIdx.x; i < 3 * NE * NP; i += blockDim.x) sphi[i] = tab[O::PHIE + i];
   __syncthreads();
   if (ed >= m.nedge || cs < 0) return;
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   // stage 0: an error of this step's edge pass (error_s0) stops the case here and becomes its error
   const unsigned long long es0 = OP == OP_S0 ? cc.error_s0 : ~0ull;
   if (!force && !(case_runs(cc) && es0 == ~0ull)) {
-    if (OP == OP_S0 && es0 != ~0ull && cc.active && cc.error == ~0ull && (threadIdx.x & 63) == 0)
+    if (OP == OP_S0 && es0 != ~0ull && cc.active && case_error(cc) == ~0ull && (threadIdx.x & 63) == 0)
       atomicMin(&ctl.cases[cs].error, es0);
     if (OP == OP_S0 && b.pend && cc.live && e < cc.e_end) write_back_pending<T, NP>(b, e, m.nes);
     return;
@@ -629,7 +629,7 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
         } else if (OP == OP_S1) {
           const float d = 0.25f * (d0[k] + dtf * rr);
           st32(b.d + vb, ix, d);
-          st32(b.s + vb, ix, ((float)u[v][k] - d0[k]) + d);
+          st32(b.s2 + vb, ix, ((float)u[v][k] - d0[k]) + d);
         } else {
           const double d = 2.0 / 3.0 * ((double)d0[k] + dt * (double)rr);
           const double un = un0[k] + d;
@@ -683,7 +683,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // the fields of the case record this kernel uses, read first: the exit test below then waits on
   // them alone, not on the loads issued after them
   const int c_active = cc.active, c_live = cc.live, e_end = cc.e_end;
-  const unsigned long long c_error = cc.error;
+  const unsigned long long c_error = case_error(cc);
   // stage 0: an error of this step's edge pass (error_s0) stops the case here and becomes its error
   const unsigned long long c_es0 = OP == OP_S0 ? cc.error_s0 : ~0ull;
   const double dt = cc.dt;
@@ -877,7 +877,370 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         // u^(2) shadow = u^(1) shadow - d1 + d2 (= s^n + d2 up to float rounding); no s^n re-read
         const float d = 0.25f * (d0[v][r] + dtf * rr);
         st32(b.d + vb, ix, d);
-        st32(b.s + vb, ix, (us[v][r] - d0[v][r]) + d);
+        st32(b.s2 + vb, ix, (us[v][r] - d0[v][r]) + d);
+      } else {
+        const double d = 2.0 / 3.0 * ((double)d0[v][r] + dt * (double)rr);
+        const double un = un0[v][r] + d;
+        st32(b.un + vb, ix, un);
+        st32(b.sn + vb, ix, (float)un);
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused edge-and-element pass of a march stage (FP32-bulk, matrix cores): one wavefront per group of 16
+// elements evaluates the numerical fluxes of the group's distinct edges into LDS, then runs the element
+// pass of k_element_mfma on them. Nothing goes through the point-flux array, and the edge work is
+// spread over the wavefront's 64 lanes by (edge, side, variable) for the traces and by (edge, point)
+// for the fluxes, so the lanes stay busy whatever the group's edge count.
+//  1. traces: item (j, side, v) is variable v of the left or right element of edge j at the edge points,
+//     the operations of k_surface's edge_traces in the same order (a rebuild still pending from the
+//     limiter pass before is read instead of the stored modes, as there);
+//  2. fluxes: item (j, q) is point q of edge j: HLLC, the slip wall or the characteristic far field
+//     (src/spatial_disc/flux.rs:28-153, src/spatial_disc/boundary_condition.rs:9-97) times the half
+//     length, as k_surface evaluates them, written over the left trace of the same point;
+//  3. the element pass (volume product, lift from LDS, mass inverse, RK stage) of k_element_mfma.
+// An edge shared by two groups is evaluated by both from the same inputs with the same operations, so
+// both elements receive the same flux, bit for bit equal to the separate edge pass's; the pass reads
+// only its stage's input arrays (which the stage does not write) for the traces of other groups'
+// elements. A failing edge is recorded into CaseCtl::error_f[stage], which no wavefront of the same
+// pass reads: every wavefront of a case takes the same decision to run.
+// ---------------------------------------------------------------------------------------
+template <int NE>
+struct FusedWave {
+  float4 tr[GRP_EDGES][2][NE];  // traces of the left / right element at the edge points; the flux of a
+                                // point then replaces its left trace
+  int4 rec[GRP_EDGES];          // the group's edge records (grp_rec, grp_geo)
+  float4 geo[GRP_EDGES];
+  unsigned char err[GRP_EDGES][NE];
+};
+
+// LDS of one block of the fused pass: a FusedWave per wavefront, then the edge basis table
+template <int NP, int NE>
+constexpr size_t fused_lds_bytes(int waves) {
+  return (size_t)waves * sizeof(FusedWave<NE>) + (size_t)3 * NE * NP * sizeof(float);
+}
+
+// completes a wavefront's LDS writes before its lanes read each other's
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NP, int NQ, int NE, int OP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_fused(DevMesh m, StageBufs<float> b,
+                                                                                       const float* __restrict__ tab,
+                                                                                       Ctl ctl, int force, int remap) {
+  static_assert(NP <= 16 && NQ <= 16 && 3 * NE <= 16, "matrix-core element kernel needs p <= 3");
+  static_assert(OP != OP_RESID, "the fused pass is a march stage");
+  static_assert(GRP_EDGES <= 64, "one edge record per lane");
+  using O = TabOff<NP, NQ, NE>;
+  constexpr int STAGE = OP - OP_S0;
+  extern __shared__ __align__(16) unsigned char fused_lds[];
+  const int wpb = (int)(blockDim.x >> 6);
+  FusedWave<NE>& sw = reinterpret_cast<FusedWave<NE>*>(fused_lds)[threadIdx.x >> 6];
+  float* sphi = reinterpret_cast<float*>(fused_lds + (size_t)wpb * sizeof(FusedWave<NE>));
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl.count[STAGE] = 0;
+  const int blk = remap ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = (blk * (int)blockDim.x + (int)threadIdx.x) >> 6;  // the group: elements 16 wave ..
+  const int nes = m.nes;
+  // First, in one trip, everything that depends on the group alone (every index is in range: the
+  // tables have the arena's capacity): the case, the edge count and records, the edge basis table, and
+  // the element pass's face slots, metric and pending mark. The case record follows the case id.
+  const int cs = case_of(m, m.el_case64, wave * 16);
+  const int n = __builtin_amdgcn_readfirstlane(m.grp_n[wave]);
+  const int4 rec = m.grp_rec[(size_t)wave * GRP_EDGES + (lane < GRP_EDGES ? lane : 0)];
+  const float4 geo = m.grp_geo[(size_t)wave * GRP_EDGES + (lane < GRP_EDGES ? lane : 0)];
+  constexpr int TN = 3 * NE * NP, TI = (TN + 63) / 64;  // table entries, and per thread at most
+  float tphi[TI];
+#pragma unroll
+  for (int i = 0; i < TI; ++i) {
+    const int ix = (int)threadIdx.x + (int)blockDim.x * i;
+    tphi[i] = ix < TN ? tab[O::PHIE + ix] : 0.f;
+  }
+  const int c = lane & 15, g = lane >> 4;
+  const int e = wave * 16 + c;
+  const int es = e < m.ne ? e : 0;
+  int gsl[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int mp = 4 * g + s;
+    gsl[s] = ld32(m.el_gslot, (unsigned)((mp < 3 * NE ? mp / NE : 0) * nes + es));
+  }
+  const float* elg = m.el_geo32;
+  const unsigned eu = (unsigned)es;
+  const float xx = ld32(elg + (size_t)G_XX * nes, eu), xy = ld32(elg + (size_t)G_XY * nes, eu);
+  const float ex = ld32(elg + (size_t)G_EX * nes, eu), ey = ld32(elg + (size_t)G_EY * nes, eu);
+  const float j2 = ld32(elg + (size_t)G_2J * nes, eu);
+  const int pend_slot = b.pend ? ld32(b.pend, eu) : -1;
+  const CaseCtl& cc = ctl.cases[cs < 0 ? 0 : cs];
+  const int c_active = cc.active, c_live = cc.live, e_end = cc.e_end, ed_off = cc.ed_off;
+  unsigned long long c_error = cc.error;
+  // the fused passes of the earlier stages of this step, never this one's own word
+  if (STAGE >= 1) c_error = cc.error_f[0] < c_error ? cc.error_f[0] : c_error;
+  if (STAGE >= 2) c_error = cc.error_f[1] < c_error ? cc.error_f[1] : c_error;
+  const double dt = cc.dt;
+  // the table, and the edge records for the lanes of the trace and flux items
+#pragma unroll
+  for (int i = 0; i < TI; ++i) {
+    const int ix = (int)threadIdx.x + (int)blockDim.x * i;
+    if (ix < TN) sphi[ix] = tphi[i];
+  }
+  if (lane < GRP_EDGES) {
+    sw.rec[lane] = rec;
+    sw.geo[lane] = geo;
+  }
+  __syncthreads();  // the only block barrier, before any exit
+  if (cs < 0) return;  // the wavefront's 16 elements lie in one 64-element block, hence in one case
+  const bool runs = (force != 0) | ((c_active != 0) & (c_error == ~0ull));  // case_runs()
+  if (!runs) {
+    // a stopped case writes back the rebuilds still pending from its last limiter pass (stage 0)
+    const int ew = wave * 16 + lane;
+    if (OP == OP_S0 && b.pend && c_live && lane < 16 && ew < m.ne && ew < e_end) write_back_pending<float, NP>(b, ew, nes);
+    return;
+  }
+  // ---- 1. traces: items t = (v * 2 + side) * n + j, so that the lanes of one load cover consecutive
+  // edges of one side and variable (t / n by a multiply, exact for t < 8 * GRP_EDGES) ----
+  // (in two halves of three 64-item rounds, each half's loads issued together)
+  constexpr int R = 3;
+  static_assert(2 * R * 64 >= 8 * GRP_EDGES, "two halves cover every trace item");
+  const unsigned mn = (65536u + (unsigned)n - 1u) / (unsigned)(n > 0 ? n : 1);
+#pragma unroll 1
+  for (int h = 0; h < 2 && 64 * R * h < 8 * n; ++h) {
+    float cv[R][NP];
+    int xs[R], ps[R], its[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      xs[r] = -1;
+      ps[r] = -1;
+      its[r] = 0;
+      if (64 * (R * h + r) < 8 * n) {  // uniform
+        const int t = lane + 64 * (R * h + r);
+        const int vs = (int)(((unsigned)t * mn) >> 16), j = t - vs * n;
+        const bool item = t < 8 * n;
+        const int4 er = sw.rec[item ? j : 0];
+        const int x = item ? ((vs & 1) ? er.y : er.x) : -1;
+        const int xa = x >= 0 ? x : 0;
+        const int v = (vs >> 1) & 3;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) cv[r][k] = ld32(b.in, (unsigned)((v * NP + k) * nes + xa));
+        xs[r] = x;
+        ps[r] = b.pend ? b.pend[xa] : -1;
+        its[r] = (vs << 8) | j;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (xs[r] >= 0) {
+        const int j = its[r] & 255, vs = its[r] >> 8, side = vs & 1, v = (vs >> 1) & 3;
+        const int info = sw.rec[j].z;
+        const int lic = side ? (info >> 2) & 3 : info & 3;
+        float tr[NE];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) tr[q] = 0.f;
+        if (ps[r] >= 0) {
+          // a rebuild still pending from the limiter pass before (rare)
+          const double* w = b.pw + (size_t)ps[r] * 4 * NP + v * NP;
+#pragma unroll
+          for (int k = 0; k < NP; ++k) {
+            const float cw = (float)w[k];
+#pragma unroll
+            for (int q = 0; q < NE; ++q) tr[q] += cw * sphi[(lic * NE + (side ? NE - 1 - q : q)) * NP + k];
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < NP; ++k) {
+#pragma unroll
+            for (int q = 0; q < NE; ++q) tr[q] += cv[r][k] * sphi[(lic * NE + (side ? NE - 1 - q : q)) * NP + k];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < NE; ++q) reinterpret_cast<float*>(&sw.tr[j][side][q])[v] = tr[q];
+      }
+    }
+  }
+  // the element pass's state entries (lane: element c, modes 4g + s) and table operands go out here, to
+  // arrive while the fluxes are computed
+  float us[4][4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int k = 4 * g + s;
+      const float x = ld32(b.in + (size_t)v * NP * nes, (unsigned)((k < NP ? k : NP - 1) * nes + es));
+      us[v][s] = keep_if(k < NP, x);
+    }
+  float aPhi[4], aDxi[4], aDeta[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = 4 * g + s;
+    const int kp = k < NP ? k : NP - 1, kq = k < NQ ? k : NQ - 1;
+    const int cq = c < NQ ? c : NQ - 1, cp = c < NP ? c : NP - 1;
+    const float ph = ld32(tab, (unsigned)(O::PHIV + cq * NP + kp));
+    const float dx = ld32(tab, (unsigned)(O::DXI + kq * NP + cp)), de = ld32(tab, (unsigned)(O::DETA + kq * NP + cp));
+    aPhi[s] = keep_if(k < NP && c < NQ, ph);
+    aDxi[s] = keep_if(k < NQ && c < NP, dx);
+    aDeta[s] = keep_if(k < NQ && c < NP, de);
+  }
+  wave_lds_sync();
+  // ---- 2. fluxes: items t = (j, q) ----
+  {
+    const float hcr = (float)m.hcr;
+    for (int t = lane; t < NE * n; t += 64) {
+      const int j = t / NE, q = t - j * NE;
+      const int info = sw.rec[j].z;
+      const int bc = (info >> 4) & 3;
+      const float4 ge = sw.geo[j];
+      const float nx = ge.x, ny = ge.y, hl = ge.z;
+      const float4 a = sw.tr[j][0][q];
+      const float ul[4] = {a.x, a.y, a.z, a.w};
+      int st = 0;
+      float4 gq;
+      if (bc == BC_INTERIOR) {
+        const float4 r4 = sw.tr[j][1][q];
+        const float ur[4] = {r4.x, r4.y, r4.z, r4.w};
+        float f[4] = {0.f, 0.f, 0.f, 0.f};
+        st = hllc(ul, ur, nx, ny, hcr, f);
+        gq = make_float4(f[0] * hl, f[1] * hl, f[2] * hl, f[3] * hl);
+      } else if (bc == BC_WALL) {
+        const float p = (hcr - 1.f) * (ul[3] - 0.5f * dv(ul[1] * ul[1] + ul[2] * ul[2], ul[0]));
+        gq = make_float4(0.f, p * nx * hl, p * ny * hl, 0.f);
+      } else {
+        float f[4];
+        farfield_flux(ul, m.patch_q + 4 * (info >> 8), nx, ny, hcr, f);
+        gq = make_float4(f[0] * hl, f[1] * hl, f[2] * hl, f[3] * hl);
+      }
+      sw.tr[j][0][q] = gq;
+      sw.err[j][q] = (unsigned char)st;
+    }
+  }
+  wave_lds_sync();
+  // the first failing point of each failing edge, as k_surface records it
+  if (lane < n) {
+    int err = 0;
+#pragma unroll
+    for (int q = 0; q < NE; ++q) err = err ? err : (int)sw.err[lane][q];
+    if (err)
+      atomicMin(&ctl.cases[cs].error_f[STAGE],
+                ((unsigned long long)(unsigned)(rec.w - ed_off) << 8) | (unsigned long long)err);
+  }
+  // ---- 3. the element pass (k_element_mfma, with the fluxes from LDS) ----
+  float d0[4][4];
+  double un0[4][4];
+  if (OP == OP_S1 || OP == OP_S2) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 4 * g + r;
+        const unsigned ix = (unsigned)((k < NP ? k : NP - 1) * nes + es);
+        d0[v][r] = ld32(b.d + (size_t)v * NP * nes, ix);
+        if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
+      }
+  }
+  const bool valid = (e < m.ne) & (e < e_end);
+  float4 fl[4];
+  bool use[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int mp = 4 * g + s;
+    use[s] = valid && mp < 3 * NE;
+    const int slot = use[s] ? (gsl[s] >> 1) : 0;
+    const int pt = mp % NE;
+    const int src = (gsl[s] & 1) ? (NE - 1 - pt) : pt;
+    fl[s] = sw.tr[slot][0][src];
+  }
+  if (b.pend && valid) {
+    const int sl = pend_slot;
+    if (sl >= 0) {
+      const double* w = b.pw + (size_t)sl * 4 * NP;
+      constexpr int PREV = OP == OP_S0 ? 2 : (OP == OP_S1 ? 0 : 1);
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int k = 4 * g + s;
+          if (k < NP) {
+            us[v][s] = (float)w[v * NP + k];
+            store_limited<float, NP, PREV>(b, nes, e, v * NP + k, w[v * NP + k]);
+            if constexpr (OP == OP_S1 || OP == OP_S2)
+              d0[v][s] = (float)(w[v * NP + k] - b.un[(size_t)(v * NP + k) * nes + e]);
+          }
+        }
+    }
+  }
+  f32x4 uq[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    uq[v] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) uq[v] = mfma4(aPhi[s], us[v][s], uq[v]);
+  }
+  const float hcr = (float)m.hcr;
+  float ft[4][4], gt[4][4];  // [var][r]
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const float q4[4] = {uq[0][r], uq[1][r], uq[2][r], uq[3][r]};
+    const bool live = valid && (4 * g + r) < NQ;
+    float f[4], gg[4];
+    euler_flux(q4, hcr, f, gg);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      ft[v][r] = live ? xx * f[v] + xy * gg[v] : 0.f;
+      gt[v][r] = live ? ex * f[v] + ey * gg[v] : 0.f;
+    }
+  }
+  f32x4 acc[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    acc[v] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      acc[v] = mfma4(aDxi[s], ft[v][s], acc[v]);
+      acc[v] = mfma4(aDeta[s], gt[v][s], acc[v]);
+    }
+  }
+  float aLift[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = 4 * g + s;
+    const int km = k < 3 * NE ? k : 3 * NE - 1, cp = c < NP ? c : NP - 1;
+    aLift[s] = keep_if(k < 3 * NE && c < NP, ld32(tab, (unsigned)(O::LIFT + km * NP + cp)));
+  }
+  float sv[4][4];  // [s][var]
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const float sc = (gsl[s] & 1) ? j2 : -j2;
+    sv[s][0] = keep_if(use[s], sc * fl[s].x);
+    sv[s][1] = keep_if(use[s], sc * fl[s].y);
+    sv[s][2] = keep_if(use[s], sc * fl[s].z);
+    sv[s][3] = keep_if(use[s], sc * fl[s].w);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) acc[v] = mfma4(aLift[s], sv[s][v], acc[v]);
+  if (!valid) return;
+  const float dtf = (float)dt;
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = 4 * g + r;
+      if (k >= NP) continue;
+      const unsigned ix = (unsigned)(k * nes + e);
+      const size_t vb = (size_t)v * NP * nes;
+      const float rr = acc[v][r];
+      if (OP == OP_S0) {
+        const float d = dtf * rr;
+        st32(b.d + vb, ix, d);
+        st32(b.s + vb, ix, us[v][r] + d);
+      } else if (OP == OP_S1) {
+        const float d = 0.25f * (d0[v][r] + dtf * rr);
+        st32(b.d + vb, ix, d);
+        st32(b.s2 + vb, ix, (us[v][r] - d0[v][r]) + d);
       } else {
         const double d = 2.0 / 3.0 * ((double)d0[v][r] + dt * (double)rr);
         const double un = un0[v][r] + d;
@@ -946,8 +1309,8 @@ __global__ __launch_bounds__(256) void k_dt(DevMesh m, const T* __restrict__ U, 
 // Step control (src/temporal_disc.rs:22-34): count the step that just ran, take its successor's dt
 // (clamped to final_time), and decide whether it runs (step < final_step && t < final_time).
 // One wavefront per case slot i: lanes 0 .. DT_LANES - 1 read and reset the case's dt words in parallel.
-// It reads error but never error_s0, so the stage-0 edge pass, which records into error_s0, can run in
-// the same launch (k_surface) without changing what it decides.
+// It reads error and the fused passes' words (case_error) but never error_s0, so the stage-0 edge pass,
+// which records into error_s0, can run in the same launch (k_surface) without changing what it decides.
 __device__ __forceinline__ void step_control(const Ctl& ctl, int i, int lane) {
   static_assert(2 * DT_LANES == 64, "one wavefront reads the dt words of both step parities");
   CaseCtl& c = ctl.cases[i];
@@ -958,7 +1321,7 @@ __device__ __forceinline__ void step_control(const Ctl& ctl, int i, int lane) {
   const unsigned long long raw = *word;
   const int live = c.live, pending = c.pending, step0 = c.step, final_step = c.final_step, cnt2 = c.count[2];
   const double t0 = c.t, dt0 = c.dt, final_time = c.final_time;
-  const unsigned long long error = c.error;
+  const unsigned long long error = case_error(c);
   const int step = step0 + (pending ? 1 : 0);  // the step about to begin
   const bool mine = live && (lane >> 5) == (step & 1);
   const double v = mine ? __longlong_as_double((long long)raw) : 1.0e10;
@@ -1184,7 +1547,7 @@ __global__ __launch_bounds__(256) void k_kxrcf(DevMesh m, const T* __restrict__ 
     c_active = cc->active;
     e_end = cc->e_end;
     c_step = cc->step;
-    c_error = cc->error;
+    c_error = case_error(*cc);
     cfl = cc->cfl;
   });
   const bool runs = (cs >= 0) & ((force != 0) | ((c_active != 0) & (c_error == ~0ull)));  // case_runs()
@@ -1701,7 +2064,7 @@ __global__ __launch_bounds__(64) void k_limit_walk(DevMesh m, const int4* __rest
           if (!st) {
             if constexpr (DN) {
               if (lane < 4 * NP) {
-                b.s[ix] = (T)outv[lane];
+                (STAGE == 0 ? b.s : b.s2)[ix] = (T)outv[lane];
                 b.d[ix] = (T)(outv[lane] - un);
               }
             } else {
@@ -1830,6 +2193,15 @@ void go(const DeviceState& s, void (*kernel)(KArgs...), dim3 grid, dim3 block, h
   else
     hipLaunchKernelGGL(kernel, grid, block, 0, st, static_cast<KArgs>(args)...);
 }
+// the same with lds bytes of dynamic shared memory per block
+template <typename... KArgs, typename... Args>
+void go_lds(const DeviceState& s, void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t lds, hipStream_t st,
+            Args... args) {
+  if (s.prof_ev[0])
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, st, s.prof_ev[0], s.prof_ev[1], 0, static_cast<KArgs>(args)...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, static_cast<KArgs>(args)...);
+}
 
 }  // namespace
 
@@ -1859,7 +2231,7 @@ __global__ void k_converge(CaseCtl* cases, const double* __restrict__ force, con
   const double v = (force[2 * i + 1] * cpar[4 * i] - force[2 * i] * cpar[4 * i + 1]) * cpar[4 * i + 2] / chord;
   cl[i] = v;
   // a case that has not stepped yet (admitted since the last steps) is not tested
-  const bool running = c.active && c.error == ~0ull && (c.step > 0 || c.pending);
+  const bool running = c.active && case_error(c) == ~0ull && (c.step > 0 || c.pending);
   if (running && !conv[i] && fabs(v - clprev[i]) < tol) {
     conv[i] = 1;
     c.final_step = c.step + (c.pending ? 1 : 0);
@@ -1902,6 +2274,7 @@ class SolverImpl final : public Solver {
     b.us = sizeof(T) == 8 ? (T*)s.us : nullptr;
     b.sn = sizeof(T) == 4 ? (T*)s.sn : nullptr;
     b.s = sizeof(T) == 4 ? (T*)s.s : nullptr;
+    b.s2 = sizeof(T) == 4 ? (T*)s.s2 : nullptr;
     b.d = sizeof(T) == 4 ? (T*)s.d : nullptr;
     b.rout = s.rout;
     // the Jacobi sweep of a march leaves its rebuilds pending for the next stage's kernels to read
@@ -1912,11 +2285,11 @@ class SolverImpl final : public Solver {
   // stage input: FP64 reads u^n then the stage state; FP32-bulk reads the float shadows
   const T* stage_input(const DeviceState& s, int stage) const {
     if (sizeof(T) == 8) return stage == 0 ? (const T*)s.un : (const T*)s.us;
-    return stage == 0 ? (const T*)s.sn : (const T*)s.s;
+    return stage == 0 ? (const T*)s.sn : (stage == 1 ? (const T*)s.s : (const T*)s.s2);
   }
   const T* limiter_input(const DeviceState& s, int stage) const {
     if (sizeof(T) == 8) return stage == 2 ? (const T*)s.un : (const T*)s.us;
-    return stage == 2 ? (const T*)s.sn : (const T*)s.s;
+    return stage == 2 ? (const T*)s.sn : (stage == 0 ? (const T*)s.s : (const T*)s.s2);
   }
 
   // s0: the stage-0 edge pass of a march (errors into error_s0); nctl > 0: it also runs the step control
@@ -1930,9 +2303,23 @@ class SolverImpl final : public Solver {
     go(s, k_surface<T, NP, NQ, NE>, blocks_for(s.mesh.nedge, bs) + extra, bs, st, s.mesh, b.in, (T*)s.flux,
        (const T*)s.tab, s.ctl, force, s.tune.remap, b.pend, b.pw, s0, nctl);
   }
+  // the fused edge-and-element pass replaces a march stage's edge and element passes
+  static bool fused_on(const DeviceState& s) {
+    return sizeof(T) == 4 && NP <= 16 && NQ <= 16 && 3 * NE <= 16 && s.fused && s.use_mfma &&
+           s.tune.element_waves > 0 && s.mesh.grp_rec;
+  }
+  // march: the element pass of a stage, fused with its edge pass when fused_on()
   template <int OP>
-  void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force) const {
+  void element(const DeviceState& s, StageBufs<T> b, hipStream_t st, int force, bool march = false) const {
     if (s.mesh.ne <= 0) return;
+    if constexpr (sizeof(T) == 4 && NP <= 16 && NQ <= 16 && 3 * NE <= 16 && OP != OP_RESID) {
+      if (march && fused_on(s)) {
+        const int w = s.tune.element_waves;
+        go_lds(s, k_fused<NP, NQ, NE, OP>, blocks_for(s.mesh.ne, 16 * w), 64 * w, fused_lds_bytes<NP, NE>(w), st,
+               s.mesh, b, (const float*)s.tab, s.ctl, force, s.tune.remap);
+        return;
+      }
+    }
     if constexpr (sizeof(T) == 4 && NP <= 16 && NQ <= 16 && 3 * NE <= 16) {
       if (s.use_mfma && s.tune.element_waves > 0) {
         const int w = s.tune.element_waves;
@@ -1969,14 +2356,25 @@ class SolverImpl final : public Solver {
     StageBufs<T> b = bufs(s);
     b.in = stage_input(s, stage_idx);
     if (part == PART_SURFACE) {
-      surface(s, b, st, 0, stage_idx == 0 ? 1 : 0, stage_idx == 0 ? nctl : 0);
+      if (fused_on(s)) {
+        // the edges are the fused pass's: stage 0 keeps only the step control (a part timed on its own
+        // that launches nothing records its events back to back)
+        if (stage_idx == 0 && nctl > 0) {
+          go(s, k_step_begin, nctl, 64, st, s.ctl, nctl);
+        } else if (s.prof_ev[0]) {
+          (void)hipEventRecord(s.prof_ev[0], st);
+          (void)hipEventRecord(s.prof_ev[1], st);
+        }
+      } else {
+        surface(s, b, st, 0, stage_idx == 0 ? 1 : 0, stage_idx == 0 ? nctl : 0);
+      }
     } else if (part == PART_ELEMENT) {
       if (stage_idx == 0)
-        element<OP_S0>(s, b, st, 0);
+        element<OP_S0>(s, b, st, 0, true);
       else if (stage_idx == 1)
-        element<OP_S1>(s, b, st, 0);
+        element<OP_S1>(s, b, st, 0, true);
       else
-        element<OP_S2>(s, b, st, 0);
+        element<OP_S2>(s, b, st, 0, true);
     } else if (s.limiter != DGM_LIMITER_OFF) {
       if (stage_idx == 0)
         limiter_part<0>(s, st, 0, 0, part);

@@ -17,7 +17,8 @@ struct DeviceState {
   double* un = nullptr;    // u^n, FP64
   void* us = nullptr;      // FP64 mode: stage state
   void* sn = nullptr;      // FP32-bulk: float shadow of u^n
-  void* s = nullptr;       // FP32-bulk: float stage state
+  void* s = nullptr;       // FP32-bulk: float state of stage 1 (u^(1)), written by stage 0
+  void* s2 = nullptr;      // FP32-bulk: float state of stage 2 (u^(2)), written by stage 1
   void* d = nullptr;       // FP32-bulk: stage increment
   double* rout = nullptr;  // residual output
   void* flux = nullptr;
@@ -31,6 +32,7 @@ struct DeviceState {
   Ctl ctl{};               // case records (shared by the arenas of a batch) and this arena's list counters
   int limiter = 0;
   int use_mfma = 1;        // FP32-bulk element kernel on the matrix cores (p <= 3)
+  int fused = 0;           // a march's edge and element passes run as one fused pass (FP32-bulk, p = 3)
   Tuning tune{};
   std::vector<int> level_offsets;
   const int* levels = nullptr;
