@@ -194,6 +194,8 @@ def sweep(ncases, steps, seed=0):
     b = Batch(cap, precision="fp32", limiter="jacobi", max_cases=ncases)
     ids = [b.add(m, c["order"], c["mach"], c["aoa_deg"], final_step=steps) for c, m in zip(spec, meshes)]
     tuned = b.autotune(repeats=5)
+    # per-kernel times of each order's arena, from the start state (restored afterwards)
+    prof = b.profile_steps(5)
     b.synchronize()
     t0 = time.perf_counter()
     b.steps_async(steps)
@@ -204,8 +206,32 @@ def sweep(ncases, steps, seed=0):
                 for i, c, m in zip(ids, spec, meshes))
     b.close()
     ne = [int(m.el_count.sum()) for m in meshes]
+    # algorithmic bytes per order (edges of a triangulation: (3 elements + boundary edges) / 2)
+    per_order = {}
+    step_bytes = 0.0
+    for p in sorted(cap):
+        nel = sum(n for n, c in zip(ne, spec) if c["order"] == p)
+        if nel == 0:
+            continue
+        ned = sum((3 * n + int(m.eb_count.sum())) // 2 for n, c, m in zip(ne, spec, meshes) if c["order"] == p)
+        np_ = n_basis(p)
+        kb = kernel_bytes(np_, 3 if p <= 2 else 5, 4, nel, ned)
+        t = prof[p]
+        sb = stage_bytes_min(np_, True) * nel
+        step_bytes += 3 * sb
+        per_order[str(p)] = dict(
+            elements=nel, edges=ned, kernel_ms=t,
+            element_gbs=kb["element"] / (t["element_ms"] * 1e-3) / 1e9,
+            surface_gbs=kb["surface"] / (t["surface_ms"] * 1e-3) / 1e9 if t["surface_ms"] > 0 else None,
+            stage_frac=sb / ((t["surface_ms"] + t["element_ms"] + t["detect_ms"] + t["rebuild_ms"]) * 1e-3) / 1e9
+            / HBM_PEAK_GBS)
+    ms_step = 1e3 * wall / steps
+    roof = dict(bound="hbm", achieved=step_bytes / (ms_step * 1e-3) / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                basis="sum over orders of (80 Np + 53) bytes per element-stage x elements x 3 stages / ms per step")
     return dict(value=dofst / wall / 1e9, unit="GDOF·stage/s", cases=ncases, steps=steps, wall_s=wall,
-                ms_per_step=1e3 * wall / steps, elements=int(sum(ne)), elements_min=min(ne), elements_max=max(ne),
+                ms_per_step=ms_step, roofline=roof, per_order=per_order,
+                elements=int(sum(ne)), elements_min=min(ne), elements_max=max(ne),
                 per_order_elements={str(p): v for p, v in cap.items()},
                 finished_ok=int(sum(1 for i in ids if info[i]["status"] == 0 and info[i]["steps"] == steps)),
                 failed=int(sum(1 for i in ids if info[i]["status"] != 0)), tuning={str(p): t for p, t in tuned.items()},

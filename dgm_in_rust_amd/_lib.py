@@ -28,7 +28,8 @@ EXPORTED = ["dgm_ctx_create", "dgm_ctx_create_host", "dgm_ctx_destroy", "dgm_ctx
             "dgm_batch_create", "dgm_batch_destroy", "dgm_batch_last_error", "dgm_batch_footprint", "dgm_batch_add",
             "dgm_batch_remove", "dgm_batch_steps_async", "dgm_batch_synchronize", "dgm_batch_case_info",
             "dgm_batch_case_state", "dgm_batch_lift", "dgm_batch_get_tuning", "dgm_batch_set_tuning",
-            "dgm_batch_autotune", "dgm_batch_poll_async", "dgm_batch_poll_wait", "dgm_batch_device_time"]
+            "dgm_batch_autotune", "dgm_batch_poll_async", "dgm_batch_poll_wait", "dgm_batch_device_time",
+            "dgm_batch_profile_steps"]
 
 
 class Mesh(C.Structure):
@@ -160,6 +161,7 @@ def lib():
     L.dgm_batch_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(Tuning)]
     L.dgm_batch_set_tuning.argtypes = [vp, C.c_int32, C.POINTER(Tuning)]
     L.dgm_batch_autotune.argtypes = [vp, C.c_int32, C.POINTER(Tuning)]
+    L.dgm_batch_profile_steps.argtypes = [vp, C.c_int32, C.POINTER(KernelTimes)]
     L.dgm_batch_poll_async.argtypes = [vp, C.c_double, C.c_double, _i64p]
     L.dgm_batch_poll_wait.argtypes = [vp, C.c_int64, C.c_int32, C.POINTER(CaseInfo), _f64p, _i32p]
     L.dgm_batch_device_time.argtypes = [vp, _f64p]

@@ -160,6 +160,15 @@ class Batch:
         t = _lib.Tuning(**{k: int(values[k]) for k, _ in _lib.Tuning._fields_})
         self._check(self._L.dgm_batch_set_tuning(self.h, order, C.byref(t)), "dgm_batch_set_tuning")
 
+    def profile_steps(self, n: int = 5) -> Dict[int, dict]:
+        """Mean ms per launch of each order's kernels over n eager steps, the state and clocks restored
+        afterwards (dgm_batch_profile_steps): {order: {surface_ms, element_ms, detect_ms, rebuild_ms,
+        apply_ms, step_ms}} for the orders with resident cases."""
+        out = (_lib.KernelTimes * 4)()
+        self._check(self._L.dgm_batch_profile_steps(self.h, n, out), "dgm_batch_profile_steps")
+        keys = ("surface_ms", "element_ms", "detect_ms", "rebuild_ms", "apply_ms", "step_ms")
+        return {p: {k: float(getattr(out[p - 1], k)) for k in keys} for p in sorted(set(self._case_order.values()))}
+
     def autotune(self, repeats: int = 10, cache: Optional[str] = None) -> Dict[int, dict]:
         """Times the launch shapes of every order with resident cases and keeps the fastest (see
         dgm_batch_autotune). With ``cache`` (a JSON file), a result found for the same order, precision,

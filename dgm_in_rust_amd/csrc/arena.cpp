@@ -771,4 +771,78 @@ int autotune_steps(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned l
   return status;
 }
 
+int profile_step_parts(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned long long* dtmin, int ncases,
+                       hipStream_t st, int n, std::vector<std::array<double, 5>>& ms, double& step_ms,
+                       std::string& err) {
+  constexpr int NPART = Solver::N_PARTS;
+  ms.assign(arenas.size(), std::array<double, 5>{0, 0, 0, 0, 0});
+  step_ms = 0.0;
+  if (n < 1 || arenas.empty()) return DGM_OK;
+  for (Arena* A : arenas) A->solver->flush(A->S, st);
+  ACHK(hipGetLastError());
+  ACHK(hipStreamSynchronize(st));
+  StepSnapshot snap;
+  for (Arena* A : arenas) {
+    const size_t ns = (size_t)4 * A->T.np * A->cap_e;
+    if (int s = snap.add(A->S.un, ns * 8, err)) return s;
+    if (A->S.sn)
+      if (int s = snap.add(A->S.sn, ns * 4, err)) return s;
+  }
+  if (int s = snap.add(cases, sizeof(CaseCtl) * ncases, err)) return s;
+  if (int s = snap.add(dtmin, sizeof(unsigned long long) * DT_CASE * ncases, err)) return s;
+  if (int s = snap.copy(false, st, err)) return s;
+  // a start and a stop event per (arena, stage, part), and two around the step
+  std::vector<hipEvent_t> ev(2 * arenas.size() * 3 * NPART + 2, nullptr);
+  int status = DGM_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) status = DGM_HIP_ERROR;
+  const Ctl shared{cases, dtmin, nullptr};
+  auto launches = [](const Arena* A, int part) {
+    if (part <= Solver::PART_ELEMENT) return true;
+    return A->limiter == DGM_LIMITER_JACOBI && part <= Solver::PART_REBUILD;
+  };
+  for (int it = 0; it < n && !status; ++it) {
+    if (hipEventRecord(ev[ev.size() - 2], st) != hipSuccess) status = DGM_HIP_ERROR;
+    for (Arena* A : arenas)
+      if (A->limiter != DGM_LIMITER_JACOBI && A->extent_elements() > 0) A->solver->time_step(A->S, st, 0);
+    launch_step_begin(shared, ncases, st);
+    for (int s = 0; s < 3; ++s)
+      for (size_t a = 0; a < arenas.size(); ++a)
+        for (int part = 0; part < NPART; ++part) {
+          Arena* A = arenas[a];
+          if (!launches(A, part)) continue;
+          hipEvent_t* pe = &ev[2 * ((a * 3 + s) * NPART + part)];
+          if (A->extent_elements() > 0) {
+            A->S.prof_ev[0] = pe[0];
+            A->S.prof_ev[1] = pe[1];
+            A->solver->stage_part(A->S, s, part, st, 0);
+            A->S.prof_ev[0] = A->S.prof_ev[1] = nullptr;
+          } else {
+            (void)hipEventRecord(pe[0], st);
+            (void)hipEventRecord(pe[1], st);
+          }
+        }
+    if (hipEventRecord(ev[ev.size() - 1], st) != hipSuccess) status = DGM_HIP_ERROR;
+    if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) status = DGM_HIP_ERROR;
+    float x = 0.f;
+    for (size_t a = 0; a < arenas.size() && !status; ++a)
+      for (int s = 0; s < 3; ++s)
+        for (int part = 0; part < NPART; ++part) {
+          if (!launches(arenas[a], part)) continue;
+          hipEvent_t* pe = &ev[2 * ((a * 3 + s) * NPART + part)];
+          if (hipEventElapsedTime(&x, pe[0], pe[1]) != hipSuccess) status = DGM_HIP_ERROR;
+          ms[a][part] += x / (3.0 * n);
+        }
+    if (!status && hipEventElapsedTime(&x, ev[ev.size() - 2], ev[ev.size() - 1]) == hipSuccess) step_ms += x / n;
+  }
+  if (status) err = "kernel timing failed";
+  if (int s = snap.copy(true, st, err)) status = status ? status : s;
+  for (Arena* A : arenas)
+    if (hipMemsetAsync(A->S.lim_pend, 0xff, A->cap_e * 4, st) != hipSuccess && !status) status = DGM_HIP_ERROR;
+  (void)hipStreamSynchronize(st);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return status;
+}
+
 }  // namespace dgm

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -119,6 +120,14 @@ void initial_state(const Tables& T, const HostMesh& M, double density, double ao
 // so every case's state and clock are left as they were.
 int autotune_steps(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned long long* dtmin, int ncases,
                    hipStream_t st, const std::function<void()>& step, int reps, std::string& err);
+
+// Per-kernel timing of n eager steps on st: the step control once per step, then each stage's parts
+// arena by arena, each launch timed by events its own dispatch records. ms[a][part] is the mean per
+// launch of arenas[a]'s part (0 for a part that launches nothing), step_ms the mean eager step. The
+// state, case records and dt slots are restored afterwards (as by autotune_steps).
+int profile_step_parts(const std::vector<Arena*>& arenas, CaseCtl* cases, unsigned long long* dtmin, int ncases,
+                       hipStream_t st, int n, std::vector<std::array<double, 5>>& ms, double& step_ms,
+                       std::string& err);
 
 // launch shapes <-> their C-ABI record
 Tuning to_tuning(const dgm_tuning& t);

@@ -666,4 +666,33 @@ int dgm_batch_autotune(dgm_batch* b, int32_t reps, dgm_tuning chosen[4]) {
   return DGM_OK;
 }
 
+int dgm_batch_profile_steps(dgm_batch* b, int32_t n, dgm_kernel_times out[4]) {
+  if (!b || !out || n < 1) return DGM_BAD_ARGUMENT;
+  if (int s = full_sync(b)) return s;
+  std::vector<Arena*> as;
+  std::vector<int> order_of;
+  for (int g = 0; g < 4; ++g)
+    if (b->arena[g]) {
+      as.push_back(b->arena[g].get());
+      order_of.push_back(g);
+    }
+  std::vector<std::array<double, 5>> ms;
+  double step_ms = 0.0;
+  std::string err;
+  if (int s = profile_step_parts(as, b->d_cases, b->d_dtmin, b->prm.max_cases, b->stream, n, ms, step_ms, err))
+    return fail(b, s, err);
+  for (int g = 0; g < 4; ++g) std::memset(&out[g], 0, sizeof(dgm_kernel_times));
+  for (size_t a = 0; a < as.size(); ++a) {
+    dgm_kernel_times& o = out[order_of[a]];
+    o.surface_ms = ms[a][Solver::PART_SURFACE];
+    o.element_ms = ms[a][Solver::PART_ELEMENT];
+    o.detect_ms = ms[a][Solver::PART_DETECT];
+    o.rebuild_ms = ms[a][Solver::PART_REBUILD];
+    o.apply_ms = ms[a][Solver::PART_APPLY];
+    o.step_ms = step_ms;
+    o.steps = n;
+  }
+  return DGM_OK;
+}
+
 }  // extern "C"

@@ -275,6 +275,12 @@ int dgm_batch_set_tuning(dgm_batch* b, int32_t order, const dgm_tuning* t);
 /* Times the launch shapes of every served order on the resident cases (stage-0 passes only, which
  * leave every case's state and clock unchanged) and keeps the fastest; chosen[p - 1] reports them. */
 int dgm_batch_autotune(dgm_batch* b, int32_t repeats, dgm_tuning chosen[4]);
+/* Per-kernel timing of the batch (no counterpart in the reference; the batched form of
+ * dgm_profile_steps): runs n steps eagerly on the batch stream (the step control once per step, then each
+ * stage's parts order by order, each launch timed by events its own dispatch records), then restores
+ * every case's state, record and time-step slots. out[p - 1]: mean ms per launch of order p's parts and
+ * the mean eager step (all zero for an order the batch does not serve). */
+int dgm_batch_profile_steps(dgm_batch* b, int32_t n_steps, dgm_kernel_times out[4]);
 
 #ifdef __cplusplus
 }

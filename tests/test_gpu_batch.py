@@ -191,3 +191,26 @@ def test_context_autotune_leaves_results_unchanged():
     ctx.steps_async(6)
     ctx.synchronize()
     assert np.array_equal(ctx.download(), ref)
+
+
+def test_batch_profile_times_every_order_and_leaves_states_unchanged():
+    """Per-kernel timing of a batch (eager steps, events per launch) reports every served order's edge,
+    element, detector and rebuild passes, and leaves every case's state and clock as they were."""
+    steps = 12
+    b = make_batch("fp32")
+    ids = [b.add(meshgen.naca0012_ogrid(ns, nl, seed=seed), p, M, a, final_step=steps)
+           for ns, nl, seed, p, M, a in CASES]
+    b.steps_async(5)
+    before = [i["steps"] for i in b.info(16)]
+    prof = b.profile_steps(3)
+    assert set(prof) == {1, 2, 3, 4}
+    for p, t in prof.items():
+        assert t["surface_ms"] > 0 and t["element_ms"] > 0 and t["detect_ms"] > 0 and t["rebuild_ms"] > 0, (p, t)
+        assert t["step_ms"] > t["element_ms"]
+    assert [i["steps"] for i in b.info(16)] == before
+    b.steps_async(steps)
+    b.synchronize()
+    for cid, (ns, nl, seed, p, M, a) in zip(ids, CASES):
+        U, _, _ = alone(meshgen.naca0012_ogrid(ns, nl, seed=seed), p, M, a, steps, "fp32")
+        assert np.array_equal(b.state(cid), U)
+    b.close()
