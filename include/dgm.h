@@ -85,7 +85,8 @@ typedef struct {
   int64_t n_elements, n_edges, n_internal_edges, n_internal_elements, n_boundary_elements;
   int32_t n_basis, n_cell_gp, n_edge_gp, order, precision, limiter;
   int32_t n_limiter_levels;  /* level-schedule depth of the sequential sweep */
-  int32_t fused_pass;        /* 1: a march evaluates the edge fluxes inside the element pass (FP32-bulk, p = 3) */
+  int32_t fused_pass;        /* 1: a march evaluates the edge fluxes inside the element pass (FP32-bulk, p = 3,
+                                with DGM_FUSED=1 in the environment at context creation; same results) */
   double cfl;
   int64_t device_bytes;      /* device memory held by the context */
 } dgm_info;
