@@ -94,7 +94,7 @@ def run_gpu(mesh, precision, steps, warmup, order, mach, aoa, tuning=None, start
         ctx.steps_async(warmup)
         ctx.synchronize()
     if tuning is None:
-        tuning = ctx.autotune(repeats=10)   # launch shapes timed on this state (state and clock unchanged)
+        tuning = ctx.autotune(repeats=40)   # launch shapes timed on this state (state and clock unchanged)
     else:
         if tuning:
             ctx.set_tuning({**ctx.tuning(), **tuning})
