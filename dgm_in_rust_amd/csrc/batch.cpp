@@ -58,7 +58,9 @@ struct dgm_batch {
   hipEvent_t fork_ev = nullptr, join_ev[4] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int64_t captured[4][2] = {};            // arena extents the graph was captured with
+  hipGraph_t graph_n = nullptr;           // the step captured GRAPH_STEPS times: one launch per GRAPH_STEPS steps
+  hipGraphExec_t exec_n = nullptr;
+  int64_t captured[4][2] = {};            // arena extents the graphs were captured with
   dgm_error err{};
   // Admissions: a new case's mesh, state and first time step go on ustream beside the running steps; its
   // record goes live on the batch stream, between steps, once those are done (admit_ev) and only with
@@ -176,6 +178,10 @@ int full_sync(dgm_batch* b) {
   return DGM_OK;
 }
 
+// Steps per launch of the multi-step graph. Every step decides on the device, case by case, whether it
+// runs, so steps past a case's stop condition are no-ops and any number of steps may be queued.
+constexpr int GRAPH_STEPS = 8;
+
 bool graph_current(const dgm_batch* b) {
   if (!b->exec) return false;
   for (int g = 0; g < 4; ++g) {
@@ -190,8 +196,9 @@ bool graph_current(const dgm_batch* b) {
 // once the batch stream has passed them.
 void drop_graph(dgm_batch* b) {
   if (b->exec || b->graph) b->retired.push_back(Retired{b->polls, -1, 0, Placement(), b->exec, b->graph});
-  b->exec = nullptr;
-  b->graph = nullptr;
+  if (b->exec_n || b->graph_n) b->retired.push_back(Retired{b->polls, -1, 0, Placement(), b->exec_n, b->graph_n});
+  b->exec = b->exec_n = nullptr;
+  b->graph = b->graph_n = nullptr;
 }
 
 // one step of every arena on stream st (the capture stream, or the batch stream when timed eagerly)
@@ -236,6 +243,10 @@ int ensure_graph(dgm_batch* b) {
   enqueue_step(b, b->cstream);
   BCHK(b, hipStreamEndCapture(b->cstream, &b->graph));
   BCHK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  BCHK(b, hipStreamBeginCapture(b->cstream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < GRAPH_STEPS; ++i) enqueue_step(b, b->cstream);
+  BCHK(b, hipStreamEndCapture(b->cstream, &b->graph_n));
+  BCHK(b, hipGraphInstantiate(&b->exec_n, b->graph_n, nullptr, nullptr, 0));
   for (int g = 0; g < 4; ++g) {
     b->captured[g][0] = b->arena[g] ? b->arena[g]->extent_elements() : 0;
     b->captured[g][1] = b->arena[g] ? b->arena[g]->extent_edges() : 0;
@@ -496,7 +507,8 @@ int dgm_batch_steps_async(dgm_batch* b, int32_t n) {
   BCHK(b, hipEventCreate(&e0));
   BCHK(b, hipEventCreate(&e1));
   BCHK(b, hipEventRecord(e0, b->stream));
-  for (int i = 0; i < n; ++i) BCHK(b, hipGraphLaunch(b->exec, b->stream));
+  for (int i = 0; i + GRAPH_STEPS <= n; i += GRAPH_STEPS) BCHK(b, hipGraphLaunch(b->exec_n, b->stream));
+  for (int i = 0; i < n % GRAPH_STEPS; ++i) BCHK(b, hipGraphLaunch(b->exec, b->stream));
   BCHK(b, hipEventRecord(e1, b->stream));
   b->step_ev.push_back({e0, e1});
   collect_step_times(b, false);

@@ -42,6 +42,9 @@ struct dgm_ctx {
   std::vector<void*> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // the same step captured GRAPH_STEPS times in one graph: one launch per GRAPH_STEPS steps
+  hipGraph_t graph_n = nullptr;
+  hipGraphExec_t graph_exec_n = nullptr;
   dgm_error err{};
   double cfl = 0.0;
   void* scratch = nullptr;   // this device's address of the scratch array
@@ -186,6 +189,19 @@ void effective_clock(const CaseCtl& h, double& t, int32_t& steps) {
   steps = h.step + (h.pending ? 1 : 0);
 }
 
+// Steps per launch of the multi-step graph. Every step decides on the device whether it runs, so steps
+// past a stop condition are no-ops and any number of steps may be queued.
+constexpr int GRAPH_STEPS = 8;
+
+void drop_graphs(dgm_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->graph_exec_n) (void)hipGraphExecDestroy(c->graph_exec_n);
+  if (c->graph_n) (void)hipGraphDestroy(c->graph_n);
+  c->graph_exec = c->graph_exec_n = nullptr;
+  c->graph = c->graph_n = nullptr;
+}
+
 int ensure_graph(dgm_ctx* c) {
   if (!c->stream) return fail(c, DGM_BAD_ARGUMENT, "host-only context");
   if (c->graph_exec) return DGM_OK;
@@ -193,6 +209,17 @@ int ensure_graph(dgm_ctx* c) {
   enqueue_step(c);
   HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph));
   HIPCHK(c, hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < GRAPH_STEPS; ++i) enqueue_step(c);
+  HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph_n));
+  HIPCHK(c, hipGraphInstantiate(&c->graph_exec_n, c->graph_n, nullptr, nullptr, 0));
+  return DGM_OK;
+}
+
+// n steps from the captured graphs: whole multiples of GRAPH_STEPS from the multi-step graph
+int launch_steps(dgm_ctx* c, int32_t n) {
+  for (int32_t i = 0; i + GRAPH_STEPS <= n; i += GRAPH_STEPS) HIPCHK(c, hipGraphLaunch(c->graph_exec_n, c->stream));
+  for (int32_t i = 0; i < n % GRAPH_STEPS; ++i) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
   return DGM_OK;
 }
 
@@ -300,8 +327,7 @@ int dgm_ctx_create_host(const dgm_mesh* mesh, const dgm_params* params, dgm_ctx*
 void dgm_ctx_destroy(dgm_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
+  drop_graphs(c);
   c->A.reset();
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->h_ctl) (void)hipHostFree(c->h_ctl);
@@ -560,12 +586,12 @@ int dgm_device_time_march(dgm_ctx* c, int32_t final_step, double final_time, dgm
   if (!c) return DGM_BAD_ARGUMENT;
   if (int s = ensure_graph(c)) return s;
   if (int s = start_run(c, final_step, final_time)) return s;
-  const int batch = 16;
+  const int batch = 2 * GRAPH_STEPS;
   CaseCtl h;
   double t = 0.0;
   int32_t steps = 0;
   while (true) {
-    for (int i = 0; i < batch; ++i) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+    if (int s = launch_steps(c, batch)) return s;
     if (int s = read_ctl(c, h)) return s;
     effective_clock(h, t, steps);
     if (case_error(h) != ~0ull || !h.active || steps >= final_step || !(t < final_time)) break;
@@ -590,8 +616,7 @@ int dgm_time_march(dgm_ctx* c, double* U, int32_t final_step, double final_time,
 int dgm_device_steps_async(dgm_ctx* c, int32_t n) {
   if (!c) return DGM_BAD_ARGUMENT;
   if (int s = ensure_graph(c)) return s;
-  for (int i = 0; i < n; ++i) HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
-  return DGM_OK;
+  return launch_steps(c, n);
 }
 
 int dgm_synchronize(dgm_ctx* c) {
@@ -740,10 +765,7 @@ int dgm_set_tuning(dgm_ctx* c, const dgm_tuning* t) {
   if (!c || !t || !c->A || !valid_tuning(*t)) return DGM_BAD_ARGUMENT;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->A->S.tune = to_tuning(*t);
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->graph_exec = nullptr;
-  c->graph = nullptr;
+  drop_graphs(c);
   return DGM_OK;
 }
 
@@ -752,10 +774,7 @@ int dgm_autotune(dgm_ctx* c, int32_t reps, dgm_tuning* chosen) {
   std::string e;
   if (int s = autotune_steps({c->A.get()}, c->d_case, c->d_dtmin, 1, c->stream, [c]() { enqueue_step(c); }, reps, e))
     return fail(c, s, e);
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->graph_exec = nullptr;
-  c->graph = nullptr;
+  drop_graphs(c);
   if (chosen) *chosen = from_tuning(c->A->S.tune);
   CaseCtl h;
   if (int s = read_ctl(c, h)) return s;

@@ -105,6 +105,10 @@ __device__ __forceinline__ void euler_flux(const T q[4], T hcr, T f[4], T g[4]) 
 // pressure; the pressure test runs in FP64 whatever T is.
 template <class T>
 __device__ __forceinline__ int hllc(const T lv[4], const T rv[4], T nx, T ny, T hcr, T out[4]) {
+  // multiply-adds are fused within one expression only (decided in the front end), so every kernel that
+  // inlines the edge arithmetic (the edge pass, the fused edge-and-element pass) evaluates the same
+  // operations
+#pragma clang fp contract(on)
   const T ql0 = lv[0], ql1 = lv[1] * nx + lv[2] * ny, ql2 = -lv[1] * ny + lv[2] * nx, ql3 = lv[3];
   const T qr0 = rv[0], qr1 = rv[1] * nx + rv[2] * ny, qr2 = -rv[1] * ny + rv[2] * nx, qr3 = rv[3];
   const T il = dv(T(1), ql0), ir = dv(T(1), qr0);
@@ -162,6 +166,10 @@ __device__ __forceinline__ int hllc(const T lv[4], const T rv[4], T nx, T ny, T 
 // local_characteristics::compute_eigenmatrix, src/spatial_disc/local_characteristics.rs:4-50
 template <class T>
 __device__ __forceinline__ void eigenmatrix(const T q[4], T nx, T ny, T hcr, T L[16], T R[16]) {
+  // multiply-adds are fused within one expression only (decided in the front end), so every kernel that
+  // inlines the edge arithmetic (the edge pass, the fused edge-and-element pass) evaluates the same
+  // operations
+#pragma clang fp contract(on)
   const T ri = dv(T(1), q[0]);
   const T u = q[1] * ri, v = q[2] * ri;
   const T p = (hcr - T(1)) * (q[3] - T(0.5) * (q[1] * q[1] + q[2] * q[2]) * ri);
@@ -208,6 +216,10 @@ __device__ __forceinline__ void eigenmatrix(const T q[4], T nx, T ny, T hcr, T L
 // characteristic far-field state, src/spatial_disc/boundary_condition.rs:48-94
 template <class T>
 __device__ __forceinline__ void farfield_flux(const T lv[4], const double* qinf, T nx, T ny, T hcr, T out[4]) {
+  // multiply-adds are fused within one expression only (decided in the front end), so every kernel that
+  // inlines the edge arithmetic (the edge pass, the fused edge-and-element pass) evaluates the same
+  // operations
+#pragma clang fp contract(on)
   const T ri = dv(T(1), lv[0]);
   const T u = lv[1] * ri, v = lv[2] * ri;
   const T p = (hcr - T(1)) * (lv[3] - T(0.5) * (lv[1] * lv[1] + lv[2] * lv[2]) * ri);
@@ -352,6 +364,10 @@ __device__ __forceinline__ unsigned long long* dt_slot(const Ctl& ctl, int c, in
 // coef(v, k)
 template <class T, int NP, int NE, class F>
 __device__ __forceinline__ void edge_traces(F coef, const T* sphi, int lic, bool rev, T tr[NE][4]) {
+  // multiply-adds are fused within one expression only (decided in the front end), so every kernel that
+  // inlines the edge arithmetic (the edge pass, the fused edge-and-element pass) evaluates the same
+  // operations
+#pragma clang fp contract(on)
 #pragma unroll
   for (int q = 0; q < NE; ++q)
 #pragma unroll
@@ -389,10 +405,12 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
   const int blk = remap ? xcd_block(blockIdx.x, nblk) : (int)blockIdx.x;
   const int ed = blk * blockDim.x + threadIdx.x;
   const int nes = m.nes, est = m.ed_stride;
-  // the edge record (read at a clamped index) goes out before the table staging and its barrier
+  // the edge record (read at a clamped index) goes out before the table staging and its barrier, both
+  // element ids included (the right one read later would add a dependent trip before its modes)
   const int ee = ed < m.nedge ? ed : 0;
   const int info = m.ed_info[ee];
   const int L = m.ed_elem[ee];
+  const int R = m.ed_elem[est + ee];  // -1 on a boundary edge (not read there)
   const T* eg = geo_ed<T>(m);
   const T nx = eg[ee], ny = eg[est + ee], hl = eg[2 * est + ee];
   // and so does the case record (of case 0 for edges outside any case, which exit below)
@@ -417,7 +435,6 @@ __global__ __launch_bounds__(256) void k_surface(DevMesh m, const T* __restrict_
     edge_traces<T, NP, NE>([&](int v, int k) { return (T)pw[(size_t)pl * 4 * NP + v * NP + k]; }, sphi, licL, false, ul);
   T* g = G + (size_t)ed * NE * 4;
   if (bc == BC_INTERIOR) {
-    const int R = m.ed_elem[est + ed];
     const int pr = pend ? pend[R] : -1;
     T ur[NE][4];
     edge_traces<T, NP, NE>([&](int v, int k) { return U[(size_t)(v * NP + k) * nes + R]; }, sphi, licR, true, ur);
