@@ -1,4 +1,5 @@
-"""Builds dgm_in_rust_amd/libdgm.so (gfx950) from csrc/ with hipcc, in-tree.
+"""Builds dgm_in_rust_amd/libdgm.so (gfx950) from csrc/ with hipcc, in-tree, and beside it
+libdgm_study.so, the launch-boundary study's kernels (csrc/study_sync.hip; not part of the library).
 
 Every HIP translation unit is compiled with packed-FP32 instructions disabled
 (-Xclang -target-feature -Xclang -packed-fp32-ops). Objects go to dgm_in_rust_amd/_build/.
@@ -22,6 +23,8 @@ HIPFLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "
             "-munsafe-fp-atomics"]
 SOURCES = [("kernels.hip", True), ("dgm.cpp", True), ("arena.cpp", True), ("batch.cpp", True), ("tables.cpp", False),
            ("mesh.cpp", False)]
+STUDY_SOURCE = "study_sync.hip"
+STUDY_LIB = os.path.join(HERE, "libdgm_study.so")
 
 
 def _compile(src: str, hip: bool) -> str:
@@ -42,18 +45,26 @@ def _compile(src: str, hip: bool) -> str:
     return obj
 
 
-def build(verbose: bool = False) -> str:
-    os.makedirs(OUT, exist_ok=True)
-    with ThreadPoolExecutor(max_workers=4) as ex:
-        objs = list(ex.map(lambda a: _compile(*a), SOURCES))
-    tmp = LIB + ".tmp"
+def _link(objs, lib: str) -> None:
+    tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OUT, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        jobs = [ex.submit(_compile, *a) for a in SOURCES]
+        study = ex.submit(_compile, STUDY_SOURCE, True)
+        objs = [j.result() for j in jobs]
+        study_obj = study.result()
+    _link(objs, LIB)
+    _link([study_obj], STUDY_LIB)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", STUDY_LIB)
     return LIB
 
 

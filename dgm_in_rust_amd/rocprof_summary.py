@@ -23,7 +23,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak bandwidth, GB/s
 
 def kernel_key(name: str) -> str:
     """Kernel name without namespace, argument list and spaces: k_surface<float,10,16,5>."""
-    n = re.sub(r"dgm::\(anonymous namespace\)::", "", name)
+    n = re.sub(r"(dgm::)?\(anonymous namespace\)::", "", name)
     n = re.sub(r"\(dgm::.*|\(.*", "", n).replace("void ", "").replace(" ", "")
     return n[:64]
 

@@ -15,6 +15,14 @@ walk's final state is from the per-level one.
 
 python -m dgm_in_rust_amd.studies --ogrid [--steps 500] [--modes walk,levels,jacobi] [--precisions fp64,fp32]
                                   [--grid 160x50] [--order 2] [--mach 0.5]
+
+Launch-boundary study (--sync): what separating two passes costs as a boundary between launches of a
+captured graph against a grid-wide barrier (cooperative_groups grid.sync()) inside one cooperative
+launch, for light passes (one float per thread) of 256 threads per block and several grid sizes. For
+each size it times `phases` passes and one pass both ways and reports the cost of one separation,
+(T(phases) - T(1)) / (phases - 1), from the kernels of libdgm_study.so.
+
+python -m dgm_in_rust_amd.studies --sync [--blocks 256,512,1024,2048] [--phases 9] [--reps 200]
 """
 from __future__ import annotations
 
@@ -90,6 +98,41 @@ def _order_rows(mesh, steps, modes, precisions, order, mach, rows):
     return rows
 
 
+def sync_study(blocks_list, phases: int, reps: int, threads: int = 256):
+    """Cost of one separation between passes: a launch boundary inside a graph and a grid barrier inside a
+    cooperative launch (libdgm_study.so). Grid sizes that cannot be co-resident are reported and skipped."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdgm_study.so"))
+    lib.dgm_study_sync.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    lib.dgm_study_max_grid.argtypes = [C.c_int]
+    maxg = lib.dgm_study_max_grid(threads)
+    rows = []
+    for nb in blocks_list:
+        if nb > maxg:
+            print(json.dumps(dict(blocks=nb, skipped=f"more than the {maxg} co-resident blocks")), flush=True)
+            continue
+        t = {}
+        for ph in (1, phases):
+            g, c = C.c_double(), C.c_double()
+            st = lib.dgm_study_sync(nb, threads, ph, reps, C.byref(g), C.byref(c))
+            if st:
+                raise RuntimeError(f"dgm_study_sync({nb}, {threads}, {ph}, {reps}) returned {st}")
+            t[ph] = (g.value * 1e3, c.value * 1e3)   # microseconds
+        row = dict(blocks=nb, threads=threads, phases=phases, reps=reps, max_coresident_blocks=maxg,
+                   graph_us_1=t[1][0], graph_us_n=t[phases][0], grid_us_1=t[1][1], grid_us_n=t[phases][1],
+                   launch_boundary_us=(t[phases][0] - t[1][0]) / (phases - 1),
+                   grid_barrier_us=(t[phases][1] - t[1][1]) / (phases - 1))
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+    print(f"\n| Blocks (x {threads} threads) | {phases} passes in a graph (us) | one launch boundary (us) | "
+          f"{phases} passes in one cooperative launch (us) | one grid barrier (us) |")
+    print("|---|---|---|---|---|")
+    for r in rows:
+        print(f"| {r['blocks']} | {r['graph_us_n']:.1f} | {r['launch_boundary_us']:.1f} | {r['grid_us_n']:.1f} | "
+              f"{r['grid_barrier_us']:.1f} |")
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--sizes", default="64,128,256,512")
@@ -102,7 +145,14 @@ def main(argv=None):
     ap.add_argument("--grid", default="160x50", help="O-grid surface points x layers (--ogrid)")
     ap.add_argument("--order", type=int, default=2, help="polynomial order (--ogrid)")
     ap.add_argument("--mach", type=float, default=0.5, help="freestream Mach number (--ogrid)")
+    ap.add_argument("--sync", action="store_true", help="the launch-boundary study instead of the vortex")
+    ap.add_argument("--blocks", default="256,512,1024,2048", help="grid sizes (--sync)")
+    ap.add_argument("--phases", type=int, default=9, help="passes per measurement (--sync)")
+    ap.add_argument("--reps", type=int, default=200, help="repetitions per measurement (--sync)")
     a = ap.parse_args(argv)
+    if a.sync:
+        sync_study([int(x) for x in a.blocks.split(",")], a.phases, a.reps)
+        return
     if a.ogrid:
         ns, nl = (int(x) for x in a.grid.split("x"))
         ogrid_order_study(a.steps, a.modes.split(","), a.precisions.split(","), ns, nl, a.order, a.mach)
