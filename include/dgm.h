@@ -162,7 +162,9 @@ int dgm_time_march(dgm_ctx* ctx, double* solutions, int32_t final_step, double f
 int dgm_compute_residuals_device_noalloc(dgm_ctx* ctx, const double* solutions_dev, double* residuals_dev);
 int dgm_state_upload(dgm_ctx* ctx, const double* solutions);
 int dgm_state_download(dgm_ctx* ctx, double* solutions);
-/* time_march on the resident state: steps run as replays of a captured HIP graph of one RK3 step. */
+/* time_march on the resident state: steps run as replays of captured HIP graphs (eight RK3 steps per
+ * launch, then single steps; each step tests on the device whether it runs, so steps past the stop
+ * condition change nothing). */
 int dgm_device_time_march(dgm_ctx* ctx, int32_t final_step, double final_time, dgm_march_stats* stats);
 /* enqueue n whole steps (time-step reduction + 3 stages + limiter) without synchronising. */
 int dgm_device_steps_async(dgm_ctx* ctx, int32_t n_steps);
@@ -252,7 +254,7 @@ int dgm_batch_footprint(const dgm_batch* b, int32_t order, int64_t n_elements, i
 int dgm_batch_add(dgm_batch* b, const dgm_mesh* mesh, const dgm_case_params* params, const double* U0,
                   int32_t* case_id);
 int dgm_batch_remove(dgm_batch* b, int32_t case_id);
-/* enqueues n steps of every running case (one captured graph per step); returns at once */
+/* enqueues n steps of every running case (captured graphs of eight steps, then of one); returns at once */
 int dgm_batch_steps_async(dgm_batch* b, int32_t n_steps);
 int dgm_batch_synchronize(dgm_batch* b);
 /* control records of slots 0 .. n-1 (one device read) */
