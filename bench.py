@@ -56,8 +56,10 @@ def kernel_bytes(np_, ne_pts, tsize, nel, nedge):
     """Algorithmic bytes per launch: every array the kernel must read or write, each counted once.
 
     element: stage input 4Np*T per element; the point fluxes of every edge once, nedge*Ne*4*T (an
-    interior edge's flux serves both neighbours); metric and 2/J 5*T and three edge ids 3*4 per
-    element; the RK stage combination averaged over the three stages, per element
+    interior edge's flux serves both neighbours), and in FP32-bulk the edge's normal and half length
+    3*4 (the residual sums' deviation form subtracts the mean flux's normal component); metric and 2/J
+    5*T and three edge ids 3*4 per element; the RK stage combination averaged over the three stages,
+    per element
       FP64:      write u^(1); read u^n, write u^(2); read u^n, write u^(n+1)   -> (1 + 2 + 2) * 4Np*8 / 3
       FP32-bulk: write d, s; read d, write d, s; read d and u^n (FP64), write u^n (FP64) and s^n
                                                           -> (2*4 + 3*4 + (4 + 8 + 8 + 4)) * 4Np / 3.
@@ -70,7 +72,7 @@ def kernel_bytes(np_, ne_pts, tsize, nel, nedge):
         rk = (n * 8 + 2 * n * 8 + 2 * n * 8) / 3.0
     else:
         rk = (2 * n * 4 + 3 * n * 4 + (n * 4 + n * 8 + n * 8 + n * 4)) / 3.0
-    element = nel * (n * tsize + 5 * tsize + 12 + rk) + nedge * ne_pts * 4 * tsize
+    element = nel * (n * tsize + 5 * tsize + 12 + rk) + nedge * (ne_pts * 4 * tsize + (12 if tsize == 4 else 0))
     surface = nel * n * tsize + nedge * (12 + 3 * tsize + ne_pts * 4 * tsize)
     return {"element": element, "surface": surface}
 

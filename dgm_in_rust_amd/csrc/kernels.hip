@@ -163,6 +163,23 @@ __device__ __forceinline__ int hllc(const T lv[4], const T rv[4], T nx, T ny, T 
   return 0;
 }
 
+// FP32-bulk residual in deviation form. The element passes subtract from every point flux of an element
+// the flux of the element's mean state (its mode 0: the Dubiner mode 0 is 1 everywhere), at the volume
+// points and, as its normal component times the half length, at the edge points. In exact arithmetic
+// the residual is unchanged: the quadrature rules integrate a constant flux against the basis gradients
+// and against the traces exactly, so its volume and surface terms cancel (a uniform state gives R = 0
+// term by term). In FP32 the volume and lift sums then accumulate deviations from the mean flux rather
+// than full fluxes that largely cancel, so their rounding acts on the smaller numbers. Multiply-adds are
+// fused within one expression only, so the separate and the fused element passes round alike.
+__device__ __forceinline__ float dev_metric(float a, float b, float f, float g, float fr, float gr) {
+#pragma clang fp contract(on)
+  return a * (f - fr) + b * (g - gr);
+}
+__device__ __forceinline__ float dev_surface(float sc, float flux, float fr, float gr, float nx, float ny, float hl) {
+#pragma clang fp contract(on)
+  return sc * (flux - (fr * nx + gr * ny) * hl);
+}
+
 // local_characteristics::compute_eigenmatrix, src/spatial_disc/local_characteristics.rs:4-50
 template <class T>
 __device__ __forceinline__ void eigenmatrix(const T q[4], T nx, T ny, T hcr, T L[16], T R[16]) {
@@ -528,6 +545,14 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   const T ex = ld32(eg + (size_t)G_EX * nes, eu), ey = ld32(eg + (size_t)G_EY * nes, eu);
   const T j2 = ld32(eg + (size_t)G_2J * nes, eu);
   const T hcr = (T)m.hcr;
+  // FP32-bulk: the flux of the element's mean state (mode 0), subtracted from every point flux of the
+  // element (deviation form, see dev_metric); FP64 keeps the reference's plain sums
+  constexpr bool DEV = sizeof(T) == 4;
+  T fr[4] = {T(0), T(0), T(0), T(0)}, gr[4] = {T(0), T(0), T(0), T(0)};
+  if constexpr (DEV) {
+    const T mean[4] = {u[0][0], u[1][0], u[2][0], u[3][0]};
+    euler_flux(mean, hcr, fr, gr);
+  }
   T r[4][NP];
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -548,8 +573,14 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
     euler_flux(w, hcr, f, g);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const T ft = xx * f[v] + xy * g[v];
-      const T gt = ex * f[v] + ey * g[v];
+      T ft, gt;
+      if constexpr (DEV) {
+        ft = dev_metric(xx, xy, f[v], g[v], fr[v], gr[v]);
+        gt = dev_metric(ex, ey, f[v], g[v], fr[v], gr[v]);
+      } else {
+        ft = xx * f[v] + xy * g[v];
+        gt = ex * f[v] + ey * g[v];
+      }
 #pragma unroll
       for (int k = 0; k < NP; ++k) r[v][k] += ft * tab[O::DXI + q * NP + k] + gt * tab[O::DETA + q * NP + k];
     }
@@ -561,12 +592,24 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
     const int edge = es >> 1, side = es & 1;
     const unsigned gb = (unsigned)edge * (unsigned)(NE * 4);
     const T sg = side ? j2 : -j2;
+    // the edge's normal and half length, for the mean flux's surface term (FP32-bulk)
+    float fnx = 0.f, fny = 0.f, fhl = 0.f;
+    if constexpr (DEV) {
+      fnx = ld32(m.ed_geo32, (unsigned)edge);
+      fny = ld32(m.ed_geo32 + m.ed_stride, (unsigned)edge);
+      fhl = ld32(m.ed_geo32 + 2 * (size_t)m.ed_stride, (unsigned)edge);
+    }
 #pragma unroll
     for (int q = 0; q < NE; ++q) {
       const int src = side ? (NE - 1 - q) : q;
       T gv[4];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) gv[v] = sg * ld32(G, gb + src * 4 + v);
+      for (int v = 0; v < 4; ++v) {
+        if constexpr (DEV)
+          gv[v] = dev_surface(sg, ld32(G, gb + src * 4 + v), fr[v], gr[v], fnx, fny, fhl);
+        else
+          gv[v] = sg * ld32(G, gb + src * 4 + v);
+      }
 #pragma unroll
       for (int v = 0; v < 4; ++v)
 #pragma unroll
@@ -748,21 +791,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const float ex = ld32(eg + (size_t)G_EX * nes, eu), ey = ld32(eg + (size_t)G_EY * nes, eu);
   const float j2 = ld32(eg + (size_t)G_2J * nes, eu);
   const int pend_slot = (OP != OP_RESID && b.pend) ? ld32(b.pend, eu) : -1;
-  // the RK operands of the lane's entries go out with the rest, so the epilogue waits on no load of its
-  // own; a pending write-back below rewrites d of its element and patches d0 with the value it stores
+  // the RK operands (d, and u^n in stage 3) load after the volume product, below: held from here they
+  // would take the registers the deviation form's mean flux needs (a pending write-back stores d first,
+  // and the loads then read it)
   float d0[4][4];
   double un0[4][4];
-  if (OP == OP_S1 || OP == OP_S2) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = 4 * g + r;
-        const unsigned ix = (unsigned)((k < NP ? k : NP - 1) * nes + es);
-        d0[v][r] = ld32(b.d + (size_t)v * NP * nes, ix);
-        if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
-      }
-  }
   const bool runs = (force != 0) | ((c_active != 0) & (c_error == ~0ull) & (c_es0 == ~0ull));  // case_runs()
   if (!runs) {
     if (OP == OP_S0 && c_es0 != ~0ull && c_active && c_error == ~0ull && lane == 0)
@@ -788,6 +821,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int src = (ed & 1) ? (NE - 1 - pt) : pt;
     fl[s] = ld32(reinterpret_cast<const float4*>(G), (unsigned)((ed >> 1) * NE + src));
   }
+  // normal and half length of the (at most two) faces of the lane's slots, for the mean flux's surface
+  // term: face a of slot 0, face b of slot 3
+  float gxa, gya, gha, gxb, gyb, ghb;
+  {
+    const unsigned ea = (unsigned)(((e < m.ne) ? eid[0] : 0) >> 1);
+    const unsigned eb = (unsigned)(((e < m.ne && 4 * g + 3 < 3 * NE) ? eid[3] : 0) >> 1);
+    gxa = ld32(m.ed_geo32, ea);
+    gya = ld32(m.ed_geo32 + m.ed_stride, ea);
+    gha = ld32(m.ed_geo32 + 2 * (size_t)m.ed_stride, ea);
+    gxb = ld32(m.ed_geo32, eb);
+    gyb = ld32(m.ed_geo32 + m.ed_stride, eb);
+    ghb = ld32(m.ed_geo32 + 2 * (size_t)m.ed_stride, eb);
+  }
   // a rebuild still pending from the limiter pass before: use it and write it back, each lane the
   // modes it holds (rare)
   if (OP != OP_RESID && b.pend && valid) {
@@ -803,9 +849,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (k < NP) {
             us[v][s] = (float)w[v * NP + k];
             store_limited<float, NP, PREV>(b, nes, e, v * NP + k, w[v * NP + k]);
-            // the d just stored (store_limited, stages 0 and 1: d = w - u^n)
-            if constexpr (OP == OP_S1 || OP == OP_S2)
-              d0[v][s] = (float)(w[v * NP + k] - b.un[(size_t)(v * NP + k) * nes + e]);
           }
         }
     }
@@ -819,6 +862,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int s = 0; s < 4; ++s) uq[v] = mfma4(aPhi[s], us[v][s], uq[v]);
   }
   const float hcr = (float)m.hcr;
+  // the flux of the element's mean state (mode 0 sits in lane c, the g = 0 lane of element c), subtracted
+  // from every point flux of the element (deviation form, see dev_metric)
+  float fr[4], gr[4];
+  {
+    float mean[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mean[v] = __shfl(us[v][0], c, 64);
+    euler_flux(mean, hcr, fr, gr);
+  }
   // fluxes at the lane's four points, in the metric-transformed form
   float ft[4][4], gt[4][4];  // [var][r]
 #pragma unroll
@@ -829,8 +881,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     euler_flux(q4, hcr, f, gg);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      ft[v][r] = live ? xx * f[v] + xy * gg[v] : 0.f;
-      gt[v][r] = live ? ex * f[v] + ey * gg[v] : 0.f;
+      ft[v][r] = live ? dev_metric(xx, xy, f[v], gg[v], fr[v], gr[v]) : 0.f;
+      gt[v][r] = live ? dev_metric(ex, ey, f[v], gg[v], fr[v], gr[v]) : 0.f;
     }
   }
   f32x4 acc[4];
@@ -842,6 +894,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       acc[v] = mfma4(aDxi[s], ft[v][s], acc[v]);
       acc[v] = mfma4(aDeta[s], gt[v][s], acc[v]);
     }
+  }
+  if (OP == OP_S1 || OP == OP_S2) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 4 * g + r;
+        const unsigned ix = (unsigned)((k < NP ? k : NP - 1) * nes + es);
+        d0[v][r] = ld32(b.d + (size_t)v * NP * nes, ix);
+        if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
+      }
   }
   // lift operands: K index = face * NE + point
   float aLift[4];
@@ -855,10 +918,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const float sc = (eid[s] & 1) ? j2 : -j2;
-    sv[s][0] = keep_if(use[s], sc * fl[s].x);
-    sv[s][1] = keep_if(use[s], sc * fl[s].y);
-    sv[s][2] = keep_if(use[s], sc * fl[s].z);
-    sv[s][3] = keep_if(use[s], sc * fl[s].w);
+    // slot s lies on face a when it shares slot 0's face, else on face b
+    const bool fa = (4 * g + s) / NE == (4 * g) / NE;
+    const float fnx = fa ? gxa : gxb, fny = fa ? gya : gyb, fhl = fa ? gha : ghb;
+    sv[s][0] = keep_if(use[s], dev_surface(sc, fl[s].x, fr[0], gr[0], fnx, fny, fhl));
+    sv[s][1] = keep_if(use[s], dev_surface(sc, fl[s].y, fr[1], gr[1], fnx, fny, fhl));
+    sv[s][2] = keep_if(use[s], dev_surface(sc, fl[s].z, fr[2], gr[2], fnx, fny, fhl));
+    sv[s][3] = keep_if(use[s], dev_surface(sc, fl[s].w, fr[3], gr[3], fnx, fny, fhl));
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -1144,19 +1210,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 ((unsigned long long)(unsigned)(rec.w - ed_off) << 8) | (unsigned long long)err);
   }
   // ---- 3. the element pass (k_element_mfma, with the fluxes from LDS) ----
+  // the RK operands load after the volume product, as in k_element_mfma
   float d0[4][4];
   double un0[4][4];
-  if (OP == OP_S1 || OP == OP_S2) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int k = 4 * g + r;
-        const unsigned ix = (unsigned)((k < NP ? k : NP - 1) * nes + es);
-        d0[v][r] = ld32(b.d + (size_t)v * NP * nes, ix);
-        if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
-      }
-  }
   const bool valid = (e < m.ne) & (e < e_end);
   float4 fl[4];
   bool use[4];
@@ -1182,8 +1238,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
           if (k < NP) {
             us[v][s] = (float)w[v * NP + k];
             store_limited<float, NP, PREV>(b, nes, e, v * NP + k, w[v * NP + k]);
-            if constexpr (OP == OP_S1 || OP == OP_S2)
-              d0[v][s] = (float)(w[v * NP + k] - b.un[(size_t)(v * NP + k) * nes + e]);
           }
         }
     }
@@ -1196,6 +1250,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int s = 0; s < 4; ++s) uq[v] = mfma4(aPhi[s], us[v][s], uq[v]);
   }
   const float hcr = (float)m.hcr;
+  // the mean flux of k_element_mfma's deviation form, from the same values
+  float fr[4], gr[4];
+  {
+    float mean[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mean[v] = __shfl(us[v][0], c, 64);
+    euler_flux(mean, hcr, fr, gr);
+  }
   float ft[4][4], gt[4][4];  // [var][r]
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -1205,8 +1267,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     euler_flux(q4, hcr, f, gg);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      ft[v][r] = live ? xx * f[v] + xy * gg[v] : 0.f;
-      gt[v][r] = live ? ex * f[v] + ey * gg[v] : 0.f;
+      ft[v][r] = live ? dev_metric(xx, xy, f[v], gg[v], fr[v], gr[v]) : 0.f;
+      gt[v][r] = live ? dev_metric(ex, ey, f[v], gg[v], fr[v], gr[v]) : 0.f;
     }
   }
   f32x4 acc[4];
@@ -1219,6 +1281,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       acc[v] = mfma4(aDeta[s], gt[v][s], acc[v]);
     }
   }
+  if (OP == OP_S1 || OP == OP_S2) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 4 * g + r;
+        const unsigned ix = (unsigned)((k < NP ? k : NP - 1) * nes + es);
+        d0[v][r] = ld32(b.d + (size_t)v * NP * nes, ix);
+        if (OP == OP_S2) un0[v][r] = ld32(b.un + (size_t)v * NP * nes, ix);
+      }
+  }
   float aLift[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
@@ -1230,10 +1303,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const float sc = (gsl[s] & 1) ? j2 : -j2;
-    sv[s][0] = keep_if(use[s], sc * fl[s].x);
-    sv[s][1] = keep_if(use[s], sc * fl[s].y);
-    sv[s][2] = keep_if(use[s], sc * fl[s].z);
-    sv[s][3] = keep_if(use[s], sc * fl[s].w);
+    // the edge's normal and half length: lane `slot` holds the group's record of that edge
+    const int slot = use[s] ? (gsl[s] >> 1) : 0;
+    const float fnx = __shfl(geo.x, slot, 64), fny = __shfl(geo.y, slot, 64), fhl = __shfl(geo.z, slot, 64);
+    sv[s][0] = keep_if(use[s], dev_surface(sc, fl[s].x, fr[0], gr[0], fnx, fny, fhl));
+    sv[s][1] = keep_if(use[s], dev_surface(sc, fl[s].y, fr[1], gr[1], fnx, fny, fhl));
+    sv[s][2] = keep_if(use[s], dev_surface(sc, fl[s].z, fr[2], gr[2], fnx, fny, fhl));
+    sv[s][3] = keep_if(use[s], dev_surface(sc, fl[s].w, fr[3], gr[3], fnx, fny, fhl));
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)

@@ -19,9 +19,9 @@ def test_kernel_bytes_count_every_array_once():
     # surface: each element's modes once, per edge two ids + info, normal and half length, and the point fluxes
     assert kb["surface"] == nel * n * 4 + nedge * (12 + 12 + ne_pts * 16)
     # element (FP32-bulk): state, metric + 2/J, three edge ids, the RK traffic averaged over the three stages,
-    # and every edge's point fluxes once
+    # and every edge's point fluxes, normal and half length once
     rk = (2 * n * 4 + 3 * n * 4 + (n * 4 + n * 8 + n * 8 + n * 4)) / 3.0
-    assert kb["element"] == pytest.approx(nel * (n * 4 + 20 + 12 + rk) + nedge * ne_pts * 16)
+    assert kb["element"] == pytest.approx(nel * (n * 4 + 20 + 12 + rk) + nedge * (ne_pts * 16 + 12))
     kb64 = bench.kernel_bytes(np_, ne_pts, 8, nel, nedge)
     assert kb64["surface"] == nel * n * 8 + nedge * (12 + 24 + ne_pts * 32)
 
