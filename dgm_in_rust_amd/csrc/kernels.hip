@@ -175,10 +175,12 @@ __device__ __forceinline__ float dev_metric(float a, float b, float f, float g, 
 #pragma clang fp contract(on)
   return a * (f - fr) + b * (g - gr);
 }
-__device__ __forceinline__ float dev_surface(float sc, float flux, float fr, float gr, float nx, float ny, float hl) {
+// the mean flux's normal component times the half length of an edge: what dev_surface subtracts
+__device__ __forceinline__ float mean_normal_flux(float fr, float gr, float nx, float ny, float hl) {
 #pragma clang fp contract(on)
-  return sc * (flux - (fr * nx + gr * ny) * hl);
+  return (fr * nx + gr * ny) * hl;
 }
+__device__ __forceinline__ float dev_surface(float sc, float flux, float rn) { return sc * (flux - rn); }
 
 // local_characteristics::compute_eigenmatrix, src/spatial_disc/local_characteristics.rs:4-50
 template <class T>
@@ -606,7 +608,7 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         if constexpr (DEV)
-          gv[v] = dev_surface(sg, ld32(G, gb + src * 4 + v), fr[v], gr[v], fnx, fny, fhl);
+          gv[v] = dev_surface(sg, ld32(G, gb + src * 4 + v), mean_normal_flux(fr[v], gr[v], fnx, fny, fhl));
         else
           gv[v] = sg * ld32(G, gb + src * 4 + v);
       }
@@ -914,17 +916,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int km = k < 3 * NE ? k : 3 * NE - 1, cp = c < NP ? c : NP - 1;
     aLift[s] = keep_if(k < 3 * NE && c < NP, ld32(tab, (unsigned)(O::LIFT + km * NP + cp)));
   }
+  // the mean flux's normal component times the half length on faces a and b
+  float rna[4], rnb[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    rna[v] = mean_normal_flux(fr[v], gr[v], gxa, gya, gha);
+    rnb[v] = mean_normal_flux(fr[v], gr[v], gxb, gyb, ghb);
+  }
   float sv[4][4];  // [s][var]
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const float sc = (eid[s] & 1) ? j2 : -j2;
     // slot s lies on face a when it shares slot 0's face, else on face b
     const bool fa = (4 * g + s) / NE == (4 * g) / NE;
-    const float fnx = fa ? gxa : gxb, fny = fa ? gya : gyb, fhl = fa ? gha : ghb;
-    sv[s][0] = keep_if(use[s], dev_surface(sc, fl[s].x, fr[0], gr[0], fnx, fny, fhl));
-    sv[s][1] = keep_if(use[s], dev_surface(sc, fl[s].y, fr[1], gr[1], fnx, fny, fhl));
-    sv[s][2] = keep_if(use[s], dev_surface(sc, fl[s].z, fr[2], gr[2], fnx, fny, fhl));
-    sv[s][3] = keep_if(use[s], dev_surface(sc, fl[s].w, fr[3], gr[3], fnx, fny, fhl));
+    sv[s][0] = keep_if(use[s], dev_surface(sc, fl[s].x, fa ? rna[0] : rnb[0]));
+    sv[s][1] = keep_if(use[s], dev_surface(sc, fl[s].y, fa ? rna[1] : rnb[1]));
+    sv[s][2] = keep_if(use[s], dev_surface(sc, fl[s].z, fa ? rna[2] : rnb[2]));
+    sv[s][3] = keep_if(use[s], dev_surface(sc, fl[s].w, fa ? rna[3] : rnb[3]));
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -1306,10 +1314,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // the edge's normal and half length: lane `slot` holds the group's record of that edge
     const int slot = use[s] ? (gsl[s] >> 1) : 0;
     const float fnx = __shfl(geo.x, slot, 64), fny = __shfl(geo.y, slot, 64), fhl = __shfl(geo.z, slot, 64);
-    sv[s][0] = keep_if(use[s], dev_surface(sc, fl[s].x, fr[0], gr[0], fnx, fny, fhl));
-    sv[s][1] = keep_if(use[s], dev_surface(sc, fl[s].y, fr[1], gr[1], fnx, fny, fhl));
-    sv[s][2] = keep_if(use[s], dev_surface(sc, fl[s].z, fr[2], gr[2], fnx, fny, fhl));
-    sv[s][3] = keep_if(use[s], dev_surface(sc, fl[s].w, fr[3], gr[3], fnx, fny, fhl));
+    sv[s][0] = keep_if(use[s], dev_surface(sc, fl[s].x, mean_normal_flux(fr[0], gr[0], fnx, fny, fhl)));
+    sv[s][1] = keep_if(use[s], dev_surface(sc, fl[s].y, mean_normal_flux(fr[1], gr[1], fnx, fny, fhl)));
+    sv[s][2] = keep_if(use[s], dev_surface(sc, fl[s].z, mean_normal_flux(fr[2], gr[2], fnx, fny, fhl)));
+    sv[s][3] = keep_if(use[s], dev_surface(sc, fl[s].w, mean_normal_flux(fr[3], gr[3], fnx, fny, fhl)));
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
