@@ -101,6 +101,9 @@ def run_gpu(mesh, precision, steps, warmup, order, mach, aoa, tuning=None, start
         if tuning:
             ctx.set_tuning({**ctx.tuning(), **tuning})
         tuning = ctx.tuning()
+    # the step graphs are captured and uploaded before the timer starts (set-up, not step work)
+    ctx.steps_async(0)
+    ctx.synchronize()
     t0 = time.perf_counter()
     ctx.steps_async(steps)
     ctx.synchronize()
@@ -198,6 +201,7 @@ def sweep(ncases, steps, seed=0):
     tuned = b.autotune(repeats=5)
     # per-kernel times of each order's arena, from the start state (restored afterwards)
     prof = b.profile_steps(5)
+    b.steps_async(0)   # captures the step graphs before the timer starts
     b.synchronize()
     t0 = time.perf_counter()
     b.steps_async(steps)

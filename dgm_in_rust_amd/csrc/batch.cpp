@@ -247,6 +247,9 @@ int ensure_graph(dgm_batch* b) {
   for (int i = 0; i < GRAPH_STEPS; ++i) enqueue_step(b, b->cstream);
   BCHK(b, hipStreamEndCapture(b->cstream, &b->graph_n));
   BCHK(b, hipGraphInstantiate(&b->exec_n, b->graph_n, nullptr, nullptr, 0));
+  // both graphs go to the device now, so that their first launches carry no one-time cost
+  BCHK(b, hipGraphUpload(b->exec, b->stream));
+  BCHK(b, hipGraphUpload(b->exec_n, b->stream));
   for (int g = 0; g < 4; ++g) {
     b->captured[g][0] = b->arena[g] ? b->arena[g]->extent_elements() : 0;
     b->captured[g][1] = b->arena[g] ? b->arena[g]->extent_edges() : 0;
@@ -501,8 +504,9 @@ int dgm_batch_steps_async(dgm_batch* b, int32_t n) {
   bool any = false;
   for (int g = 0; g < 4; ++g) any = any || (b->arena[g] && b->arena[g]->extent_elements() > 0);
   if (int s = flush_admissions(b)) return s;
-  if (!any || n == 0) return DGM_OK;
+  if (!any) return DGM_OK;
   if (int s = ensure_graph(b)) return s;
+  if (n <= 0) return DGM_OK;  // n = 0: the step graphs are captured, nothing is launched
   hipEvent_t e0, e1;
   BCHK(b, hipEventCreate(&e0));
   BCHK(b, hipEventCreate(&e1));

@@ -213,6 +213,9 @@ int ensure_graph(dgm_ctx* c) {
   for (int i = 0; i < GRAPH_STEPS; ++i) enqueue_step(c);
   HIPCHK(c, hipStreamEndCapture(c->stream, &c->graph_n));
   HIPCHK(c, hipGraphInstantiate(&c->graph_exec_n, c->graph_n, nullptr, nullptr, 0));
+  // both graphs go to the device now, so that their first launches carry no one-time cost
+  HIPCHK(c, hipGraphUpload(c->graph_exec, c->stream));
+  HIPCHK(c, hipGraphUpload(c->graph_exec_n, c->stream));
   return DGM_OK;
 }
 

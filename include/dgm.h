@@ -166,7 +166,8 @@ int dgm_state_download(dgm_ctx* ctx, double* solutions);
  * launch, then single steps; each step tests on the device whether it runs, so steps past the stop
  * condition change nothing). */
 int dgm_device_time_march(dgm_ctx* ctx, int32_t final_step, double final_time, dgm_march_stats* stats);
-/* enqueue n whole steps (time-step reduction + 3 stages + limiter) without synchronising. */
+/* enqueue n whole steps (time-step reduction + 3 stages + limiter) without synchronising; n = 0 only
+ * captures the step graphs (set-up that a later call would otherwise do first). */
 int dgm_device_steps_async(dgm_ctx* ctx, int32_t n_steps);
 int dgm_synchronize(dgm_ctx* ctx);
 /* the context's HIP stream (hipStream_t), for event timing by the caller */
@@ -254,7 +255,8 @@ int dgm_batch_footprint(const dgm_batch* b, int32_t order, int64_t n_elements, i
 int dgm_batch_add(dgm_batch* b, const dgm_mesh* mesh, const dgm_case_params* params, const double* U0,
                   int32_t* case_id);
 int dgm_batch_remove(dgm_batch* b, int32_t case_id);
-/* enqueues n steps of every running case (captured graphs of eight steps, then of one); returns at once */
+/* enqueues n steps of every running case (captured graphs of eight steps, then of one); returns at once;
+ * n = 0 only captures the step graphs */
 int dgm_batch_steps_async(dgm_batch* b, int32_t n_steps);
 int dgm_batch_synchronize(dgm_batch* b);
 /* control records of slots 0 .. n-1 (one device read) */
