@@ -136,3 +136,45 @@ def test_product_tables_match_oracle(tiny_mesh, p, nq, ne):
         assert np.array_equal(t[k], ref[k]), k
     for k in ("cell_w", "phi_cell", "phi_edge", "mass"):
         assert np.allclose(t[k], ref[k], rtol=2e-15, atol=2e-15), k
+
+
+def constant_flux_gap(t, md, fx, fy, cell_w_scale=1.0):
+    """Volume minus surface term of a constant flux (fx, fy) against every basis function on the
+    reference triangle, from the given quadrature and edge-basis tables: 0.5 sum_q w_q (fx dphi/dxi + fy
+    dphi/deta) - sum_faces |e|/2 sum_i w_i phi(face, i) (F . n). The gradients are exact (fractions)."""
+    faces = [((0.0, -1.0), 1.0), ((1.0 / np.sqrt(2.0), 1.0 / np.sqrt(2.0)), np.sqrt(2.0)), ((-1.0, 0.0), 1.0)]
+    gap = np.zeros(len(md))
+    scale = np.zeros(len(md))
+    for b, (l, k) in enumerate(md):
+        vol = 0.0
+        for q, (xi, eta) in enumerate(t["cell_pts"]):
+            w = t["cell_w"][q] * cell_w_scale
+            vol += 0.5 * w * (fx * X.eval_deriv(l, k, 1, 0, xi, eta) + fy * X.eval_deriv(l, k, 0, 1, xi, eta))
+        sur = 0.0
+        for f, ((nx, ny), le) in enumerate(faces):
+            for i in range(len(t["edge_w"])):
+                sur += 0.5 * le * t["edge_w"][i] * t["phi_edge"][f, i, b] * (fx * nx + fy * ny)
+        gap[b] = vol - sur
+        scale[b] = abs(vol) + abs(sur) + 1.0
+    return gap, scale
+
+
+@pytest.mark.parametrize("p,nq,ne", [(1, 6, 3), (2, 6, 3), (3, 16, 5), (4, 25, 5)])
+def test_constant_flux_volume_equals_surface(tiny_mesh, p, nq, ne):
+    """The FP32-bulk element passes subtract each element's mean flux from its point fluxes (deviation
+    form); that leaves the residual unchanged only if, for a constant flux, the volume term equals the
+    surface term for every basis function, as the divergence theorem gives and as the quadrature rules
+    reproduce when they integrate the gradients (degree p - 1) and the traces (degree p) exactly. Checked
+    on the product's tables for every supported rule; the 6-point rule's 14-digit literals leave about
+    1e-14. A rule whose weights sum to 1/2 (as the reference's 4-point rule's do) would break it: scaling
+    the weights so fails the check."""
+    ctx = Context(tiny_mesh, order=p, n_cell_gp=nq, n_edge_gp=ne, host_only=True)
+    t = ctx.tables()
+    md = X.modes(p)
+    rng = np.random.default_rng(p)
+    for _ in range(3):
+        fx, fy = rng.uniform(-3.0, 3.0, size=2)
+        gap, scale = constant_flux_gap(t, md, fx, fy)
+        assert np.all(np.abs(gap) <= 1e-13 * scale), gap
+        bad, bscale = constant_flux_gap(t, md, fx, fy, cell_w_scale=0.5)
+        assert np.any(np.abs(bad) > 1e-3 * bscale), bad
