@@ -163,24 +163,71 @@ __device__ __forceinline__ int hllc(const T lv[4], const T rv[4], T nx, T ny, T 
   return 0;
 }
 
-// FP32-bulk residual in deviation form. The element passes subtract from every point flux of an element
-// the flux of the element's mean state (its mode 0: the Dubiner mode 0 is 1 everywhere), at the volume
-// points and, as its normal component times the half length, at the edge points. In exact arithmetic
-// the residual is unchanged: the quadrature rules integrate a constant flux against the basis gradients
-// and against the traces exactly, so its volume and surface terms cancel (a uniform state gives R = 0
-// term by term). In FP32 the volume and lift sums then accumulate deviations from the mean flux rather
-// than full fluxes that largely cancel, so their rounding acts on the smaller numbers. Multiply-adds are
-// fused within one expression only, so the separate and the fused element passes round alike.
-__device__ __forceinline__ float dev_metric(float a, float b, float f, float g, float fr, float gr) {
+// FP32-bulk residual in deviation form. The element passes subtract the flux of the element's mean state
+// qm (its mode 0: the Dubiner mode 0 is 1 everywhere) from every point flux of the element: at the volume
+// points as F(qm + dq) - F(qm), evaluated from the point's deviation dq (the sum over the modes above mode
+// 0) by flux_deviation, and at the edge points as its normal component times the half length
+// (surface_deviation). In exact arithmetic the residual is unchanged: the quadrature rules integrate a
+// constant flux against the basis gradients and against the traces exactly, so its volume and surface
+// terms cancel. In FP32 the point fluxes, their products with the tables and the sums then round at the
+// scale of the deviations rather than of the full fluxes, which largely cancel in a smooth flow. What the
+// volume terms leave out is the exact flux of qm, so the surface terms subtract it in FP64, where its
+// rounding stays far below FP32's. Multiply-adds are fused within one expression only, so the separate
+// and the fused element passes round alike.
+struct MeanState {
+  float u, v, h;  // velocity and total enthalpy per volume (E + p) of the mean state
+};
+__device__ __forceinline__ MeanState mean_state(const float qm[4], float hcr) {
 #pragma clang fp contract(on)
-  return a * (f - fr) + b * (g - gr);
+  const float ri = dv(1.f, qm[0]);
+  const float u = qm[1] * ri, v = qm[2] * ri;
+  const float p = (hcr - 1.f) * (qm[3] - 0.5f * (qm[1] * qm[1] + qm[2] * qm[2]) * ri);
+  return {u, v, qm[3] + p};
 }
-// the mean flux's normal component times the half length of an edge: what dev_surface subtracts
-__device__ __forceinline__ float mean_normal_flux(float fr, float gr, float nx, float ny, float hl) {
+// F(qm + dq) - F(qm) and G(qm + dq) - G(qm) of the Euler flux (euler_flux) from the identities, with
+// r' = r + dr the point's density, u' = u + du its velocity and (m, n) = r (u, v):
+//   du = (dm - u dr) / r',  d(m u) = dm u' + m du,  d(m v) = dm v' + m dv,  dp = (g - 1)(dE - (d(m u) + d(n v)) / 2),
+//   d((E + p) u) = (dE + dp) u' + (E + p) du,  and likewise in y;
+// every term is a deviation times a mean quantity, so the rounding scales with the deviation
+__device__ __forceinline__ void flux_deviation(const float qm[4], const MeanState& s, const float dq[4], float hcr,
+                                               float df[4], float dg[4]) {
 #pragma clang fp contract(on)
-  return (fr * nx + gr * ny) * hl;
+  const float ri = dv(1.f, qm[0] + dq[0]);
+  const float du = (dq[1] - s.u * dq[0]) * ri, dw = (dq[2] - s.v * dq[0]) * ri;
+  const float u1 = s.u + du, v1 = s.v + dw;
+  const float dmu = dq[1] * u1 + qm[1] * du, dnv = dq[2] * v1 + qm[2] * dw;
+  const float dnu = dq[2] * u1 + qm[2] * du, dmv = dq[1] * v1 + qm[1] * dw;
+  const float dp = (hcr - 1.f) * (dq[3] - 0.5f * (dmu + dnv));
+  const float dh = dq[3] + dp;
+  df[0] = dq[1];
+  df[1] = dmu + dp;
+  df[2] = dnu;
+  df[3] = dh * u1 + s.h * du;
+  dg[0] = dq[2];
+  dg[1] = dmv;
+  dg[2] = dnv + dp;
+  dg[3] = dh * v1 + s.h * dw;
 }
-__device__ __forceinline__ float dev_surface(float sc, float flux, float rn) { return sc * (flux - rn); }
+// the metric-transformed flux a f + b g
+__device__ __forceinline__ float metric(float a, float b, float f, float g) {
+#pragma clang fp contract(on)
+  return a * f + b * g;
+}
+// the flux of the mean state in FP64
+__device__ __forceinline__ void mean_flux64(const float qm[4], double hcr, double fr[4], double gr[4]) {
+  const double q[4] = {(double)qm[0], (double)qm[1], (double)qm[2], (double)qm[3]};
+  euler_flux(q, hcr, fr, gr);
+}
+// the mean flux's normal component times the half length of an edge (FP64): what surface_deviation subtracts
+__device__ __forceinline__ double mean_normal_flux(double fr, double gr, float nx, float ny, float hl) {
+#pragma clang fp contract(on)
+  return (fr * (double)nx + gr * (double)ny) * (double)hl;
+}
+// lift operand of an edge point: sc (the side's sign times 2/J) times the point flux less the mean flux's
+// normal part, the difference taken in FP64 and rounded once
+__device__ __forceinline__ float surface_deviation(float sc, float flux, double rn) {
+  return sc * (float)((double)flux - rn);
+}
 
 // local_characteristics::compute_eigenmatrix, src/spatial_disc/local_characteristics.rs:4-50
 template <class T>
@@ -547,13 +594,15 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   const T ex = ld32(eg + (size_t)G_EX * nes, eu), ey = ld32(eg + (size_t)G_EY * nes, eu);
   const T j2 = ld32(eg + (size_t)G_2J * nes, eu);
   const T hcr = (T)m.hcr;
-  // FP32-bulk: the flux of the element's mean state (mode 0), subtracted from every point flux of the
-  // element (deviation form, see dev_metric); FP64 keeps the reference's plain sums
+  // FP32-bulk: point fluxes as deviations from the flux of the element's mean state (mode 0; deviation
+  // form, see flux_deviation); FP64 keeps the reference's plain sums
   constexpr bool DEV = sizeof(T) == 4;
-  T fr[4] = {T(0), T(0), T(0), T(0)}, gr[4] = {T(0), T(0), T(0), T(0)};
+  float qm[4] = {0.f, 0.f, 0.f, 0.f};
+  MeanState ms{0.f, 0.f, 0.f};
   if constexpr (DEV) {
-    const T mean[4] = {u[0][0], u[1][0], u[2][0], u[3][0]};
-    euler_flux(mean, hcr, fr, gr);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) qm[v] = (float)u[v][0];
+    ms = mean_state(qm, (float)hcr);
   }
   T r[4][NP];
 #pragma unroll
@@ -563,22 +612,26 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
   // volume integral with reference-element gradients: sum_q (F~ dphi/dxi + G~ dphi/deta) w_q / m_k
 #pragma unroll 2
   for (int q = 0; q < NQ; ++q) {
+    // the point value (FP32-bulk: its deviation, the modes above mode 0)
     T w[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       T s = T(0);
 #pragma unroll
-      for (int k = 0; k < NP; ++k) s += u[v][k] * tab[O::PHIV + q * NP + k];
+      for (int k = DEV ? 1 : 0; k < NP; ++k) s += u[v][k] * tab[O::PHIV + q * NP + k];
       w[v] = s;
     }
     T f[4], g[4];
-    euler_flux(w, hcr, f, g);
+    if constexpr (DEV)
+      flux_deviation(qm, ms, w, hcr, f, g);
+    else
+      euler_flux(w, hcr, f, g);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       T ft, gt;
       if constexpr (DEV) {
-        ft = dev_metric(xx, xy, f[v], g[v], fr[v], gr[v]);
-        gt = dev_metric(ex, ey, f[v], g[v], fr[v], gr[v]);
+        ft = metric(xx, xy, f[v], g[v]);
+        gt = metric(ex, ey, f[v], g[v]);
       } else {
         ft = xx * f[v] + xy * g[v];
         gt = ex * f[v] + ey * g[v];
@@ -588,6 +641,8 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
     }
   }
   // surface: gather the three edges' point fluxes (reversed point order on the right side)
+  double fr[4] = {0.0, 0.0, 0.0, 0.0}, gr[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (DEV) mean_flux64(qm, m.hcr, fr, gr);
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const int es = ld32(m.el_edge, (unsigned)(i * nes + e));
@@ -608,7 +663,7 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         if constexpr (DEV)
-          gv[v] = dev_surface(sg, ld32(G, gb + src * 4 + v), mean_normal_flux(fr[v], gr[v], fnx, fny, fhl));
+          gv[v] = surface_deviation(sg, ld32(G, gb + src * 4 + v), mean_normal_flux(fr[v], gr[v], fnx, fny, fhl));
         else
           gv[v] = sg * ld32(G, gb + src * 4 + v);
       }
@@ -783,7 +838,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int cq = c < NQ ? c : NQ - 1, cp = c < NP ? c : NP - 1;
     const float ph = ld32(tab, (unsigned)(O::PHIV + cq * NP + kp));
     const float dx = ld32(tab, (unsigned)(O::DXI + kq * NP + cp)), de = ld32(tab, (unsigned)(O::DETA + kq * NP + cp));
-    aPhi[s] = keep_if(k < NP && c < NQ, ph);
+    aPhi[s] = keep_if(k > 0 && k < NP && c < NQ, ph);  // the value product sums the modes above mode 0
     aDxi[s] = keep_if(k < NQ && c < NP, dx);
     aDeta[s] = keep_if(k < NQ && c < NP, de);
   }
@@ -855,7 +910,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
     }
   }
-  // values at the cell points q = 4g + r of element e
+  // deviations of the values at the cell points q = 4g + r of element e from its mean (mode 0 is left
+  // out of the product)
   f32x4 uq[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -864,27 +920,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int s = 0; s < 4; ++s) uq[v] = mfma4(aPhi[s], us[v][s], uq[v]);
   }
   const float hcr = (float)m.hcr;
-  // the flux of the element's mean state (mode 0 sits in lane c, the g = 0 lane of element c), subtracted
-  // from every point flux of the element (deviation form, see dev_metric)
-  float fr[4], gr[4];
-  {
-    float mean[4];
+  // the element's mean state (mode 0 sits in lane c, the g = 0 lane of element c): every point flux enters
+  // as its deviation from the mean's flux (deviation form, see flux_deviation)
+  float qm[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) mean[v] = __shfl(us[v][0], c, 64);
-    euler_flux(mean, hcr, fr, gr);
-  }
-  // fluxes at the lane's four points, in the metric-transformed form
+  for (int v = 0; v < 4; ++v) qm[v] = __shfl(us[v][0], c, 64);
+  // flux deviations at the lane's four points, in the metric-transformed form
   float ft[4][4], gt[4][4];  // [var][r]
+  {
+    const MeanState ms = mean_state(qm, hcr);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const float q4[4] = {uq[0][r], uq[1][r], uq[2][r], uq[3][r]};
-    const bool live = valid && (4 * g + r) < NQ;
-    float f[4], gg[4];
-    euler_flux(q4, hcr, f, gg);
+    for (int r = 0; r < 4; ++r) {
+      const float dq[4] = {uq[0][r], uq[1][r], uq[2][r], uq[3][r]};
+      const bool live = valid && (4 * g + r) < NQ;
+      float f[4], gg[4];
+      flux_deviation(qm, ms, dq, hcr, f, gg);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      ft[v][r] = live ? dev_metric(xx, xy, f[v], gg[v], fr[v], gr[v]) : 0.f;
-      gt[v][r] = live ? dev_metric(ex, ey, f[v], gg[v], fr[v], gr[v]) : 0.f;
+      for (int v = 0; v < 4; ++v) {
+        ft[v][r] = live ? metric(xx, xy, f[v], gg[v]) : 0.f;
+        gt[v][r] = live ? metric(ex, ey, f[v], gg[v]) : 0.f;
+      }
     }
   }
   f32x4 acc[4];
@@ -916,23 +971,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int km = k < 3 * NE ? k : 3 * NE - 1, cp = c < NP ? c : NP - 1;
     aLift[s] = keep_if(k < 3 * NE && c < NP, ld32(tab, (unsigned)(O::LIFT + km * NP + cp)));
   }
-  // the mean flux's normal component times the half length on faces a and b
-  float rna[4], rnb[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    rna[v] = mean_normal_flux(fr[v], gr[v], gxa, gya, gha);
-    rnb[v] = mean_normal_flux(fr[v], gr[v], gxb, gyb, ghb);
-  }
+  // the mean flux (FP64), whose normal component times the half length each slot subtracts
+  double fr[4], gr[4];
+  mean_flux64(qm, m.hcr, fr, gr);
   float sv[4][4];  // [s][var]
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const float sc = (eid[s] & 1) ? j2 : -j2;
     // slot s lies on face a when it shares slot 0's face, else on face b
     const bool fa = (4 * g + s) / NE == (4 * g) / NE;
-    sv[s][0] = keep_if(use[s], dev_surface(sc, fl[s].x, fa ? rna[0] : rnb[0]));
-    sv[s][1] = keep_if(use[s], dev_surface(sc, fl[s].y, fa ? rna[1] : rnb[1]));
-    sv[s][2] = keep_if(use[s], dev_surface(sc, fl[s].z, fa ? rna[2] : rnb[2]));
-    sv[s][3] = keep_if(use[s], dev_surface(sc, fl[s].w, fa ? rna[3] : rnb[3]));
+    const float fnx = fa ? gxa : gxb, fny = fa ? gya : gyb, fhl = fa ? gha : ghb;
+    sv[s][0] = keep_if(use[s], surface_deviation(sc, fl[s].x, mean_normal_flux(fr[0], gr[0], fnx, fny, fhl)));
+    sv[s][1] = keep_if(use[s], surface_deviation(sc, fl[s].y, mean_normal_flux(fr[1], gr[1], fnx, fny, fhl)));
+    sv[s][2] = keep_if(use[s], surface_deviation(sc, fl[s].z, mean_normal_flux(fr[2], gr[2], fnx, fny, fhl)));
+    sv[s][3] = keep_if(use[s], surface_deviation(sc, fl[s].w, mean_normal_flux(fr[3], gr[3], fnx, fny, fhl)));
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
@@ -1171,7 +1223,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int cq = c < NQ ? c : NQ - 1, cp = c < NP ? c : NP - 1;
     const float ph = ld32(tab, (unsigned)(O::PHIV + cq * NP + kp));
     const float dx = ld32(tab, (unsigned)(O::DXI + kq * NP + cp)), de = ld32(tab, (unsigned)(O::DETA + kq * NP + cp));
-    aPhi[s] = keep_if(k < NP && c < NQ, ph);
+    aPhi[s] = keep_if(k > 0 && k < NP && c < NQ, ph);  // the value product sums the modes above mode 0
     aDxi[s] = keep_if(k < NQ && c < NP, dx);
     aDeta[s] = keep_if(k < NQ && c < NP, de);
   }
@@ -1258,25 +1310,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int s = 0; s < 4; ++s) uq[v] = mfma4(aPhi[s], us[v][s], uq[v]);
   }
   const float hcr = (float)m.hcr;
-  // the mean flux of k_element_mfma's deviation form, from the same values
-  float fr[4], gr[4];
-  {
-    float mean[4];
+  // the deviation form of k_element_mfma, from the same values
+  float qm[4];
 #pragma unroll
-    for (int v = 0; v < 4; ++v) mean[v] = __shfl(us[v][0], c, 64);
-    euler_flux(mean, hcr, fr, gr);
-  }
+  for (int v = 0; v < 4; ++v) qm[v] = __shfl(us[v][0], c, 64);
   float ft[4][4], gt[4][4];  // [var][r]
+  {
+    const MeanState ms = mean_state(qm, hcr);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const float q4[4] = {uq[0][r], uq[1][r], uq[2][r], uq[3][r]};
-    const bool live = valid && (4 * g + r) < NQ;
-    float f[4], gg[4];
-    euler_flux(q4, hcr, f, gg);
+    for (int r = 0; r < 4; ++r) {
+      const float dq[4] = {uq[0][r], uq[1][r], uq[2][r], uq[3][r]};
+      const bool live = valid && (4 * g + r) < NQ;
+      float f[4], gg[4];
+      flux_deviation(qm, ms, dq, hcr, f, gg);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      ft[v][r] = live ? dev_metric(xx, xy, f[v], gg[v], fr[v], gr[v]) : 0.f;
-      gt[v][r] = live ? dev_metric(ex, ey, f[v], gg[v], fr[v], gr[v]) : 0.f;
+      for (int v = 0; v < 4; ++v) {
+        ft[v][r] = live ? metric(xx, xy, f[v], gg[v]) : 0.f;
+        gt[v][r] = live ? metric(ex, ey, f[v], gg[v]) : 0.f;
+      }
     }
   }
   f32x4 acc[4];
@@ -1307,6 +1358,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int km = k < 3 * NE ? k : 3 * NE - 1, cp = c < NP ? c : NP - 1;
     aLift[s] = keep_if(k < 3 * NE && c < NP, ld32(tab, (unsigned)(O::LIFT + km * NP + cp)));
   }
+  double fr[4], gr[4];
+  mean_flux64(qm, m.hcr, fr, gr);
   float sv[4][4];  // [s][var]
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
@@ -1314,10 +1367,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // the edge's normal and half length: lane `slot` holds the group's record of that edge
     const int slot = use[s] ? (gsl[s] >> 1) : 0;
     const float fnx = __shfl(geo.x, slot, 64), fny = __shfl(geo.y, slot, 64), fhl = __shfl(geo.z, slot, 64);
-    sv[s][0] = keep_if(use[s], dev_surface(sc, fl[s].x, mean_normal_flux(fr[0], gr[0], fnx, fny, fhl)));
-    sv[s][1] = keep_if(use[s], dev_surface(sc, fl[s].y, mean_normal_flux(fr[1], gr[1], fnx, fny, fhl)));
-    sv[s][2] = keep_if(use[s], dev_surface(sc, fl[s].z, mean_normal_flux(fr[2], gr[2], fnx, fny, fhl)));
-    sv[s][3] = keep_if(use[s], dev_surface(sc, fl[s].w, mean_normal_flux(fr[3], gr[3], fnx, fny, fhl)));
+    sv[s][0] = keep_if(use[s], surface_deviation(sc, fl[s].x, mean_normal_flux(fr[0], gr[0], fnx, fny, fhl)));
+    sv[s][1] = keep_if(use[s], surface_deviation(sc, fl[s].y, mean_normal_flux(fr[1], gr[1], fnx, fny, fhl)));
+    sv[s][2] = keep_if(use[s], surface_deviation(sc, fl[s].z, mean_normal_flux(fr[2], gr[2], fnx, fny, fhl)));
+    sv[s][3] = keep_if(use[s], surface_deviation(sc, fl[s].w, mean_normal_flux(fr[3], gr[3], fnx, fny, fhl)));
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
