@@ -971,20 +971,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int km = k < 3 * NE ? k : 3 * NE - 1, cp = c < NP ? c : NP - 1;
     aLift[s] = keep_if(k < 3 * NE && c < NP, ld32(tab, (unsigned)(O::LIFT + km * NP + cp)));
   }
-  // the mean flux (FP64), whose normal component times the half length each slot subtracts
+  // the mean flux (FP64), whose normal component times the half length each slot subtracts: formed per
+  // variable for the (at most two) faces a and b of the lane's slots, so that only two such values are
+  // held at a time
   double fr[4], gr[4];
   mean_flux64(qm, m.hcr, fr, gr);
   float sv[4][4];  // [s][var]
 #pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const float sc = (eid[s] & 1) ? j2 : -j2;
-    // slot s lies on face a when it shares slot 0's face, else on face b
-    const bool fa = (4 * g + s) / NE == (4 * g) / NE;
-    const float fnx = fa ? gxa : gxb, fny = fa ? gya : gyb, fhl = fa ? gha : ghb;
-    sv[s][0] = keep_if(use[s], surface_deviation(sc, fl[s].x, mean_normal_flux(fr[0], gr[0], fnx, fny, fhl)));
-    sv[s][1] = keep_if(use[s], surface_deviation(sc, fl[s].y, mean_normal_flux(fr[1], gr[1], fnx, fny, fhl)));
-    sv[s][2] = keep_if(use[s], surface_deviation(sc, fl[s].z, mean_normal_flux(fr[2], gr[2], fnx, fny, fhl)));
-    sv[s][3] = keep_if(use[s], surface_deviation(sc, fl[s].w, mean_normal_flux(fr[3], gr[3], fnx, fny, fhl)));
+  for (int v = 0; v < 4; ++v) {
+    const double rna = mean_normal_flux(fr[v], gr[v], gxa, gya, gha);
+    const double rnb = mean_normal_flux(fr[v], gr[v], gxb, gyb, ghb);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const float sc = (eid[s] & 1) ? j2 : -j2;
+      // slot s lies on face a when it shares slot 0's face, else on face b
+      const bool fa = (4 * g + s) / NE == (4 * g) / NE;
+      const float f = v == 0 ? fl[s].x : (v == 1 ? fl[s].y : (v == 2 ? fl[s].z : fl[s].w));
+      sv[s][v] = keep_if(use[s], surface_deviation(sc, f, fa ? rna : rnb));
+    }
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v)
