@@ -746,7 +746,7 @@ __global__ __launch_bounds__(256) void k_element(DevMesh m, StageBufs<T> b, cons
         } else if (OP == OP_S1) {
           const float d = 0.25f * (d0[k] + dtf * rr);
           st32(b.d + vb, ix, d);
-          st32(b.s2 + vb, ix, ((float)u[v][k] - d0[k]) + d);
+          st32(b.s2 + vb, ix, (float)u[v][k] + (d - d0[k]));  // one rounding at the state's scale
         } else {
           const double d = 2.0 / 3.0 * ((double)d0[k] + dt * (double)rr);
           const double un = un0[k] + d;
@@ -1021,10 +1021,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         st32(b.d + vb, ix, d);
         st32(b.s + vb, ix, us[v][r] + d);
       } else if (OP == OP_S1) {
-        // u^(2) shadow = u^(1) shadow - d1 + d2 (= s^n + d2 up to float rounding); no s^n re-read
+        // u^(2) shadow = u^(1) shadow + (d2 - d1) (= s^n + d2 up to float rounding); no s^n re-read. The
+        // increments' difference is formed first, so that the sum rounds once at the state's scale
         const float d = 0.25f * (d0[v][r] + dtf * rr);
         st32(b.d + vb, ix, d);
-        st32(b.s2 + vb, ix, (us[v][r] - d0[v][r]) + d);
+        st32(b.s2 + vb, ix, us[v][r] + (d - d0[v][r]));
       } else {
         const double d = 2.0 / 3.0 * ((double)d0[v][r] + dt * (double)rr);
         const double un = un0[v][r] + d;
@@ -1398,7 +1399,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
       } else if (OP == OP_S1) {
         const float d = 0.25f * (d0[v][r] + dtf * rr);
         st32(b.d + vb, ix, d);
-        st32(b.s2 + vb, ix, (us[v][r] - d0[v][r]) + d);
+        st32(b.s2 + vb, ix, us[v][r] + (d - d0[v][r]));
       } else {
         const double d = 2.0 / 3.0 * ((double)d0[v][r] + dt * (double)rr);
         const double un = un0[v][r] + d;
