@@ -6,10 +6,12 @@
 //              far-field flux (src/spatial_disc/boundary_condition.rs:9-97) on boundary edges;
 //              writes flux * |e|/2 per point. Each edge is evaluated once with the reference's
 //              left/right orientation (the altered p* makes HLLC orientation-dependent).
-//   k_element  one lane per element: volume integral (src/spatial_disc.rs:40-69) with reference-
-//              element gradients, gather of its three edges' fluxes and lift, the diagonal mass
-//              inverse (src/spatial_disc.rs:349-357), and the SSP-RK3 stage combination. The
-//              flux-to-element step is a gather, so no atomics and no edge colouring are needed.
+//   k_element  one lane per element (k_element_mfma: 16 elements per wavefront on the matrix cores):
+//              volume integral (src/spatial_disc.rs:40-69) with reference-element gradients, gather of
+//              its three edges' fluxes and lift, the diagonal mass inverse (src/spatial_disc.rs:349-357),
+//              and the SSP-RK3 stage combination. The flux-to-element step is a gather, so no atomics and
+//              no edge colouring are needed. In FP32-bulk mode the residual runs in deviation form
+//              (flux_deviation).
 //   limiter    KXRCF detector (src/spatial_disc/limiter/weno.rs:7-85), one lane per element,
 //              then one wavefront per troubled element for the characteristic WENO rebuild
 //              (weno.rs:86-269) with FP64 smoothness indicators and weights.
@@ -849,8 +851,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const float j2 = ld32(eg + (size_t)G_2J * nes, eu);
   const int pend_slot = (OP != OP_RESID && b.pend) ? ld32(b.pend, eu) : -1;
   // the RK operands (d, and u^n in stage 3) load after the volume product, below: held from here they
-  // would take the registers the deviation form's mean flux needs (a pending write-back stores d first,
-  // and the loads then read it)
+  // would take the registers the deviation form's mean state and flux deviations need (a pending
+  // write-back stores d first, and the loads then read it)
   float d0[4][4];
   double un0[4][4];
   const bool runs = (force != 0) | ((c_active != 0) & (c_error == ~0ull) & (c_es0 == ~0ull));  // case_runs()
