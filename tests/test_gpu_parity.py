@@ -226,6 +226,33 @@ def test_residual_fp32_bulk(all_meshes, name, variant):
     assert err.max() <= FP32_RESID_TOL, err
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("name,order,waves", [("periodic", 1, None), ("periodic", 3, 4), ("periodic", 3, 0),
+                                              ("periodic", 4, None), ("box", 2, None), ("box", 3, 4)])
+def test_uniform_flow_is_steady(precision, name, order, waves):
+    """A uniform state equal to the far-field state is a steady solution of the wall-free meshes (the
+    periodic square, the far-field box): its residual vanishes up to rounding. Measured per variable
+    against the residual of a 5 % perturbation of the same state. In FP32-bulk the volume terms are
+    exactly zero (every point's deviation from the element mean is zero), so what is left is the FP32
+    rounding of the edge fluxes; both element kernels at p = 3 (waves 4: matrix cores, 0: one lane).
+    Measured: at most 1.0e-11 (FP64) and 1.8e-5 (FP32-bulk), the momentum on the far-field box, whose
+    characteristic boundary flux rounds at the pressure's scale; the bounds sit ten times above that,
+    while a flux or boundary term that does not preserve the uniform state gives ratios of 1e-2 to 1."""
+    mesh = meshgen.periodic_square(6, seed=5) if name == "periodic" else meshgen.box_farfield(5, seed=6)
+    kw = dict(mach=0.5, aoa_deg=1.25)
+    ref, ctx = pair(mesh, order, limiter="off", precision=precision, **kw)
+    if waves is not None:
+        t = ctx.tuning()
+        t["element_waves"] = waves
+        ctx.set_tuning(t)
+    U = ctx.initial_solution(1.0, kw["aoa_deg"], kw["mach"], 1.0e5)
+    R = SpatialDisc(ctx).residuals(U)
+    Rp = SpatialDisc(ctx).residuals(flow_state(ref, seed=21, amp=0.05, mach=kw["mach"]))
+    ratio = np.array([np.abs(R[:, v]).max() / np.abs(Rp[:, v]).max() for v in range(4)])
+    print(f"{name} p={order} {precision} waves={waves}: uniform / perturbed residual per variable {ratio}", flush=True)
+    assert ratio.max() <= (1e-10 if precision == "fp64" else 2e-4), ratio
+
+
 @pytest.mark.parametrize("name,cfl", [("ogrid_p3", 0.3), ("ogrid_p1", 0.0), ("periodic_p4", 0.0)])
 def test_time_march_fp32_bulk(all_meshes, name, cfl):
     """20 FP32-bulk steps against the FP64 oracle (cfl 0: the reference's CFL formula for the order)."""
