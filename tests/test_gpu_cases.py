@@ -183,3 +183,53 @@ def test_ogrid_16k_walk_equals_level_launches(precision, monkeypatch):
         assert st["steps"] == 200
         out.append(U)
     assert np.array_equal(out[0], out[1])
+
+
+def test_benchmark_case_full_size():
+    """The benchmark's case at its full size: the NACA 0012 O-grid of 640 x 100 quads (128 000 triangles),
+    p = 3, M 0.8, AoA 1.25, Jacobi WENO. From the benchmark's shocked start, in FP64 against the oracle:
+    the residual, each entry within 1e-12 of the magnitude of its element's volume, edge and boundary
+    terms (the wall cells are far smaller than the shock's, so their terms, and the rounding of their
+    sums, are far larger than the largest residual; a max-norm against that residual reads about 2e-11
+    here); the limiter (the same troubled cells, the same rebuilt state) and two time steps within
+    1e-12 per variable. From the benchmark's freestream start: 20 FP32-bulk steps within 1e-6 of 20 FP64
+    steps per variable (the bench compares 2100)."""
+    import os
+    mesh = meshgen.naca0012_ogrid(640, 100, seed=0)
+    kw = dict(mach=0.8, aoa_deg=1.25)
+    c64 = Context(mesh, order=3, limiter="jacobi", **kw)
+    assert c64.ne == 128000
+    ref = RefCase(mesh, order=3, limiter=2, nthreads=min(16, os.cpu_count() or 1), **kw)
+    U0 = cases.shocked_start(c64, kw["mach"], kw["aoa_deg"])
+    Rref, s, _ = ref.compute_residuals(U0)
+    assert s == 0
+    R = SpatialDisc(c64).residuals(U0)
+    V, E, B, s = ref.terms(U0)
+    assert s == 0
+    # per element and variable, the largest mass-scaled term over the modes (mode 0's terms cancel to
+    # nothing in a uniform region; the higher modes' volume terms still carry the flux's scale there)
+    scale = ref.divide_by_mass(np.abs(V) + np.abs(E) + np.abs(B)).max(axis=2, keepdims=True)
+    dev = float((np.abs(R - Rref) / np.maximum(scale, 1e-300)).max())
+    assert dev <= 1e-12, dev
+    Lref, tref, s = ref.apply_limiter(U0)
+    assert s == 0 and tref.sum() > 100, tref.sum()
+    L = U0.copy()
+    t = np.zeros(c64.ne, np.uint8)
+    SpatialDisc(c64).apply_limiter(L, t)
+    assert np.array_equal(t.astype(bool), tref)
+    assert rel_err(L, Lref) <= 1e-12, rel_err(L, Lref)
+    Uref, s, n, _ = ref.time_march(U0.copy(), 2, 1.0e9)
+    assert s == 0 and n == 2
+    V, st = march(c64, U0, 2, 1.0e9)
+    assert st["steps"] == 2
+    print(f"128k shocked start: residual {dev:.2e} of its terms ({rel_err(R, Rref):.2e} max-norm), limiter "
+          f"{rel_err(L, Lref):.2e} "
+          f"({int(tref.sum())} troubled), two steps {rel_err(V, Uref):.2e}", flush=True)
+    assert rel_err(V, Uref) <= 1e-12, rel_err(V, Uref)
+    c32 = Context(mesh, order=3, limiter="jacobi", precision="fp32", **kw)
+    F0 = c64.initial_solution(1.0, kw["aoa_deg"], kw["mach"], 1.0e5)
+    U64, _ = march(c64, F0, 20, 1.0e9)
+    U32, _ = march(c32, F0, 20, 1.0e9)
+    err = cases.relative_l2_per_variable(c64, U32, U64)
+    print(f"128k freestream start, 20 steps: FP32-bulk vs FP64 per variable {err}", flush=True)
+    assert max(err.values()) <= 1e-6, err
