@@ -214,3 +214,49 @@ def test_batch_profile_times_every_order_and_leaves_states_unchanged():
         U, _, _ = alone(meshgen.naca0012_ogrid(ns, nl, seed=seed), p, M, a, steps, "fp32")
         assert np.array_equal(b.state(cid), U)
     b.close()
+
+
+def test_sweep_full_size_cases_match_alone():
+    """The benchmark's batched sweep at its full size: 256 cases of the sweep distribution (4.39 M elements,
+    p = 1-3, 2k-60k triangles each, FP32-bulk, Jacobi WENO) in one batch for 100 steps. The smallest and
+    the largest case of each order, and case 69 (which the bench's 1000-step sweep stops on a positivity
+    loss), equal the same cases run alone, bit for bit, with the same step count and status; every case
+    has run its 100 steps or stopped with an error status."""
+    from dgm_in_rust_amd import cases
+    steps = 100
+    spec = cases.sweep_cases(256, seed=0)
+    meshes = [meshgen.naca0012_ogrid(c["ns"], c["nl"], seed=c["mesh_seed"]) for c in spec]
+    cap = {p: 0 for p in (1, 2, 3)}
+    for c, m in zip(spec, meshes):
+        cap[c["order"]] += (int(m.el_count.sum()) + 63) // 64 * 64
+    assert sum(cap.values()) >= 4391624
+    b = Batch(cap, precision="fp32", limiter="jacobi", max_cases=256)
+    ids = [b.add(m, c["order"], c["mach"], c["aoa_deg"], final_step=steps) for c, m in zip(spec, meshes)]
+    assert None not in ids
+    b.steps_async(steps)
+    b.synchronize()
+    info = b.info(256)
+    for i in ids:
+        assert (info[i]["status"] == 0 and info[i]["steps"] == steps) or info[i]["status"] != 0, info[i]
+    sample = {69}
+    for p in (1, 2, 3):
+        sizes = [(int(m.el_count.sum()), k) for k, (c, m) in enumerate(zip(spec, meshes)) if c["order"] == p]
+        sample |= {min(sizes)[1], max(sizes)[1]}
+    for k in sorted(sample):
+        c = spec[k]
+        ctx = Context(meshes[k], order=c["order"], mach=c["mach"], aoa_deg=c["aoa_deg"], precision="fp32",
+                      limiter="jacobi")
+        U = ctx.initial_solution(1.0, c["aoa_deg"], c["mach"], 1.0e5)
+        try:
+            st = TemperalDisc(ctx, final_step=steps, final_time=1.0e300).time_march(SpatialDisc(ctx), ctx.zeros(), U)
+            status, nsteps = 0, st["steps"]
+        except Exception as err:  # a positivity loss: the march reports its status
+            status, nsteps = getattr(err, "status", -1), None
+        print(f"sweep case {k}: p={c['order']}, {int(meshes[k].el_count.sum())} triangles, batch status "
+              f"{info[ids[k]]['status']} steps {info[ids[k]]['steps']}, alone status {status}", flush=True)
+        assert info[ids[k]]["status"] == status
+        if status == 0:
+            assert info[ids[k]]["steps"] == nsteps == steps
+            assert np.array_equal(b.state(ids[k]), U), (k, np.abs(b.state(ids[k]) - U).max())
+        ctx.close()
+    b.close()
