@@ -275,10 +275,10 @@ def cpu_model():
     return None
 
 
-def cpu_baseline(mesh, order, mach, aoa, nsteps, nsteps_1t, config1_steps):
+def cpu_baseline(mesh, order, mach, aoa, nsteps, nsteps_1t, ogrid16k_steps):
     """The oracle on the host: the same 128k-triangle p = 3 case from freestream for nsteps on the host-core
-    share of one GPU (OMP_NUM_THREADS, else every core), nsteps_1t on one thread, and config 1 (160 x 50
-    O-grid, p = 2, M = 0.5, CFL 0.3, 500 steps) on the same threads. Sequential element-order WENO, as the
+    share of one GPU (OMP_NUM_THREADS, else every core), nsteps_1t on one thread, and the 16k-triangle
+    O-grid case (160 x 50, p = 2, M = 0.5, CFL 0.3, 500 steps) on the same threads. Sequential element-order WENO, as the
     reference; threads only in the three residual folds, as the reference's rayon folds."""
     from dgm_in_rust_amd import meshgen
     from oracle.refcpu import RefCase
@@ -305,10 +305,10 @@ def cpu_baseline(mesh, order, mach, aoa, nsteps, nsteps_1t, config1_steps):
                one_thread=dict(value=v1, steps=done1, seconds=sec1),
                all_threads=dict(value=v, threads=threads, steps=done, seconds=sec), cpu_model=cpu_model(),
                nproc=os.cpu_count())
-    if config1_steps > 0:
+    if ogrid16k_steps > 0:
         c1 = meshgen.naca0012_ogrid(160, 50, seed=0)
-        vc, secc, donec = rate(c1, 2, 0.5, 1.25, config1_steps, threads, cfl=0.3)
-        out["config1"] = dict(workload="naca0012_ogrid_160x50_p2_M0.5_aoa1.25_cfl0.3", steps=donec, seconds=secc,
+        vc, secc, donec = rate(c1, 2, 0.5, 1.25, ogrid16k_steps, threads, cfl=0.3)
+        out["ogrid_16k_p2"] = dict(workload="naca0012_ogrid_160x50_p2_M0.5_aoa1.25_cfl0.3", steps=donec, seconds=secc,
                               value=vc, threads=threads)
     return out
 
@@ -323,7 +323,7 @@ def main():
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps of the same case on the host-core share")
     ap.add_argument("--cpu-steps-1t", type=int, default=20, help="oracle steps of the same case on one thread")
-    ap.add_argument("--config1-steps", type=int, default=500, help="oracle steps of the 16k-triangle p = 2 case (0: skip)")
+    ap.add_argument("--ogrid16k-steps", type=int, default=500, help="oracle steps of the 16k-triangle p = 2 case (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-shock", action="store_true", help="skip the shocked-start FP32-bulk timing")
@@ -406,7 +406,7 @@ def main():
         print(f"serve fp64 rate {rates[mid]:g}: {sv['fp64'][f'{rates[mid]:g}']}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(mesh, args.order, mach, aoa, args.cpu_steps, args.cpu_steps_1t, args.config1_steps)
+        cpu = cpu_baseline(mesh, args.order, mach, aoa, args.cpu_steps, args.cpu_steps_1t, args.ogrid16k_steps)
         print(f"cpu baseline: {cpu['all_threads']['value']:.5f} GDOF*stage/s on {cpu['all_threads']['threads']} threads, "
               f"{cpu['one_thread']['value']:.5f} on one", file=sys.stderr, flush=True)
     line = {

@@ -57,7 +57,7 @@ def test_vortex_fp32_bulk_against_fp64(order):
 
 @pytest.mark.parametrize("order,n", [(1, 512), (2, 256), (2, 512), (3, 128), (3, 256), (3, 512), (4, 128), (4, 256),
                                      (4, 512)])
-def test_vortex_config2_fp32_bulk_within_bar(order, n):
+def test_vortex_fp32_bulk_within_1e6_of_fp64(order, n):
     """The isentropic vortex on the N x N square to t = 2 (the accuracy study's case): with the residual
     in deviation form (point fluxes as differences from the mean's flux) and the stage-2 shadow rounded
     once, FP32-bulk stays within 1e-6 of FP64 per variable up to N = 512 at every order (at most 7.7e-7
