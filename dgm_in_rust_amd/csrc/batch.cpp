@@ -317,7 +317,7 @@ int dgm_batch_create(const dgm_batch_params* params, dgm_batch** out) {
   std::memset(b->h_act, 0, sizeof(CaseCtl) * nc);
   std::memset(b->h_empty, 0, sizeof(CaseCtl));
   b->h_empty->error = b->h_empty->error_s0 = ~0ull;
-  for (int s = 0; s < 3; ++s) b->h_empty->error_f[s] = ~0ull;
+  for (int f = 0; f < 3; ++f) b->h_empty->error_f[f] = ~0ull;
   if ((s = balloc(b.get(), &b->d_cpar, 4 * (size_t)nc))) return cleanup(s);
   if ((s = balloc(b.get(), &b->d_clprev, nc))) return cleanup(s);
   if ((s = balloc(b.get(), &b->d_conv, nc))) return cleanup(s);

@@ -264,11 +264,11 @@ int build_device(dgm_ctx* c) {
     std::vector<int> wk;
     for (size_t l = 0; l + 1 < c->M.level_offsets.size(); ++l) {
       for (int32_t q = c->M.level_offsets[l]; q < c->M.level_offsets[l + 1]; ++q) {
-        const int e = c->M.level_elems[q];
-        wk.push_back(e);
+        const int el = c->M.level_elems[q];
+        wk.push_back(el);
         for (int i = 0; i < 3; ++i) {
-          const int64_t n = c->M.el_nbr[3 * (size_t)e + i];
-          wk.push_back(n >= 0 && n < e ? (int)n : -1);
+          const int64_t n = c->M.el_nbr[3 * (size_t)el + i];
+          wk.push_back(n >= 0 && n < el ? (int)n : -1);
         }
       }
       while (wk.size() % 256) wk.push_back(-1);
