@@ -2287,9 +2287,6 @@ __global__ void k_from_soa(const double* __restrict__ in, double* __restrict__ o
   if (e >= n) return;
   out[(size_t)e * ncomp + c] = in[(size_t)c * nes + e0 + e];
 }
-__global__ void k_copy_f64(const double* __restrict__ in, double* __restrict__ out, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = in[i];
-}
 __global__ void k_check_finite(const double* __restrict__ u, int e0, int n, int nes, int ncomp, int* flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
