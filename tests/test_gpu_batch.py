@@ -7,7 +7,6 @@ import numpy as np
 import pytest
 
 from dgm_in_rust_amd import Batch, Context, SpatialDisc, TemperalDisc, meshgen
-from dgm_in_rust_amd.batch import n_basis
 
 pytestmark = pytest.mark.gpu
 

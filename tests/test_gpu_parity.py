@@ -3,15 +3,13 @@
 Bar: FP64 mode within 1e-12 (max-norm relative per conserved variable); FP32-bulk mode within
 the tolerances stated per test. Troubled-cell sets must match exactly.
 """
-import os
-
 import numpy as np
 import pytest
 
 from dgm_in_rust_amd import Context, SpatialDisc, TemperalDisc, meshgen
 from dgm_in_rust_amd._lib import DgmError
 from oracle.refcpu import RefCase
-from tests.helpers import flow_state, rel_err, rel_l2, rel_l2_per_var
+from tests.helpers import flow_state, rel_err, rel_l2_per_var
 
 pytestmark = pytest.mark.gpu
 
