@@ -191,8 +191,8 @@ def test_benchmark_case_full_size():
     the residual, each entry within 1e-12 of the magnitude of its element's volume, edge and boundary
     terms (the wall cells are far smaller than the shock's, so their terms, and the rounding of their
     sums, are far larger than the largest residual; a max-norm against that residual reads about 2e-11
-    here); the limiter (the same troubled cells, the same rebuilt state) and two time steps within
-    1e-12 per variable; and the FP32-bulk residual within 1e-5 of its element's terms. From the
+    here); the limiter in the Jacobi order and in the reference's element order (the same troubled cells,
+    the same rebuilt state) and two time steps within 1e-12 per variable; and the FP32-bulk residual within 1e-5 of its element's terms. From the
     benchmark's freestream start: 20 FP32-bulk steps within 1e-6 of 20 FP64 steps per variable (the
     bench compares 2100)."""
     import os
@@ -219,6 +219,19 @@ def test_benchmark_case_full_size():
     SpatialDisc(c64).apply_limiter(L, t)
     assert np.array_equal(t.astype(bool), tref)
     assert rel_err(L, Lref) <= 1e-12, rel_err(L, Lref)
+    # the reference's own limiter order (the in-place element-order sweep, one walk launch at this size)
+    rseq = RefCase(mesh, order=3, limiter=1, nthreads=min(16, os.cpu_count() or 1), **kw)
+    Sref, sref, s = rseq.apply_limiter(U0)
+    assert s == 0
+    cseq = Context(mesh, order=3, limiter="sequential", **kw)
+    S = U0.copy()
+    ts = np.zeros(cseq.ne, np.uint8)
+    SpatialDisc(cseq).apply_limiter(S, ts)
+    assert np.array_equal(ts.astype(bool), sref)
+    assert rel_err(S, Sref) <= 1e-12, rel_err(S, Sref)
+    print(f"128k shocked start, element-order limiter: {int(sref.sum())} troubled, {rel_err(S, Sref):.2e}",
+          flush=True)
+    cseq.close()
     Uref, s, n, _ = ref.time_march(U0.copy(), 2, 1.0e9)
     assert s == 0 and n == 2
     V, st = march(c64, U0, 2, 1.0e9)
