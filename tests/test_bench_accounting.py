@@ -45,3 +45,17 @@ def test_summary_fractions_and_matrix_core_utilisation():
     assert m["issued_frac"] == pytest.approx(m["issued_tflops"] / 157.3)
     assert 0 < m["useful_frac"] < m["issued_frac"]
     assert s["value"] == pytest.approx(128000 * 40 * 3 * 2000 / r["wall"] / 1e9)
+
+
+def test_more_than_one_gpu_is_refused_before_any_gpu_work():
+    """`bench.py --gpus N` with N > 1 exits with status 2 and a message naming the one-GPU limit, before it
+    loads the library or touches a device."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stdout + r.stderr
+    assert r.stdout.strip() == "" or not r.stdout.strip().startswith("{")
